@@ -1,0 +1,495 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  The C sources
+restate sparkit's algorithms with the reference's operation order; each
+function below names the reference file:line it follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+OK, INVALID_ARGUMENT, LOGIC_ERROR, NO_MEMORY, BREAKDOWN = 0, 1, 2, 3, 5
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "-s"],
+                       check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class CsrD(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64),
+                ("row_ptr", C.c_void_p), ("col_ind", C.c_void_p),
+                ("vals", C.c_void_p)]
+
+
+class IluD(C.Structure):
+    _fields_ = [("L", CsrD), ("U", CsrD), ("tmp", C.c_void_p)]
+
+
+class ConfigD(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("restart", C.c_int64),
+                ("max_iterations", C.c_int64)]
+
+
+class ConfigF(C.Structure):
+    _fields_ = [("tolerance", C.c_float), ("restart", C.c_int64),
+                ("max_iterations", C.c_int64)]
+
+
+class SummaryD(C.Structure):
+    _fields_ = [("residual_norm", C.c_double),
+                ("computed_iterations", C.c_int64),
+                ("converged", C.c_int32), ("status", C.c_int32),
+                ("history_len", C.c_int64)]
+
+
+class SummaryF(C.Structure):
+    _fields_ = [("residual_norm", C.c_float),
+                ("computed_iterations", C.c_int64),
+                ("converged", C.c_int32), ("status", C.c_int32),
+                ("history_len", C.c_int64)]
+
+
+def _declare(L):
+    L.orc_csr_pattern.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p,
+                                  _i64p, _i64p, _i64p, C.POINTER(C.c_int64)]
+    L.orc_csr_pattern.restype = C.c_int
+    L.orc_make_matrix.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p,
+                                  _i64p, _f64p, _i64p, _i64p, _f64p,
+                                  C.POINTER(C.c_int64)]
+    L.orc_make_matrix.restype = C.c_int
+    L.orc_csr_at.argtypes = [_i64p, _i64p, _f64p, C.c_int64, C.c_int64]
+    L.orc_csr_at.restype = C.c_double
+    L.orc_spmv_d.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p, _f64p]
+    L.orc_spmv_f.argtypes = [C.c_int64, _i64p, _i64p, _f32p, _f32p, _f32p]
+    L.orc_extract_diagonal.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p,
+                                       _f64p, _f64p]
+    L.orc_jacobi.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _f64p, _f64p]
+    L.orc_jacobi.restype = C.c_int
+    L.orc_jacobi_apply_d.argtypes = [C.c_int64, _f64p, _f64p, _f64p]
+    L.orc_ilu0.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _f64p, C.c_int,
+                           _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]
+    L.orc_ilu0.restype = C.c_int
+    for nm, fp in (("d", _f64p), ("f", _f32p)):
+        for side in ("forward", "backward"):
+            getattr(L, f"orc_{side}_solve_{nm}").argtypes = [
+                C.c_int64, _i64p, _i64p, fp, fp, fp]
+    L.orc_true_relres_d.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p,
+                                    _f64p]
+    L.orc_true_relres_d.restype = C.c_double
+    L.orc_splitmix64_at.argtypes = [C.c_uint64, C.c_uint64]
+    L.orc_splitmix64_at.restype = C.c_uint64
+    L.orc_uniform_pm1.argtypes = [C.c_uint64, C.c_int64, _f64p]
+    for nm in ("orc_cg", "orc_gmres", "orc_bicgstab"):
+        for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
+                                   ("f", _f32p, ConfigF, SummaryF)):
+            getattr(L, f"{nm}_{sfx}").argtypes = [
+                C.c_int64, fp, fp, C.POINTER(cfg), C.c_void_p, C.c_void_p,
+                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                C.POINTER(summ), C.c_void_p]
+    for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
+                               ("f", _f32p, ConfigF, SummaryF)):
+        getattr(L, f"orc_fgmres_{sfx}").argtypes = [
+            C.c_int64, fp, fp, C.POINTER(cfg), C.c_void_p, C.c_void_p,
+            C.c_void_p, C.c_void_p, C.POINTER(summ), C.c_void_p]
+    L.orc_gen_tridiagonal.argtypes = [C.c_int64, C.c_double, C.c_double,
+                                      C.c_double, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+    L.orc_gen_tridiagonal.restype = C.c_int64
+    L.orc_gen_poisson_2d.argtypes = [C.c_int64, C.c_int64, C.c_double,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_gen_poisson_2d.restype = C.c_int64
+    L.orc_gen_convection_diffusion_2d.argtypes = [
+        C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_void_p,
+        C.c_void_p, C.c_void_p]
+    L.orc_gen_convection_diffusion_2d.restype = C.c_int64
+    L.orc_gen_convdiff_centered_2d.argtypes = [
+        C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+        C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_gen_convdiff_centered_2d.restype = C.c_int64
+    L.orc_gen_nonsymmetric_sample.argtypes = [C.c_void_p] * 3
+    L.orc_gen_nonsymmetric_sample.restype = C.c_int64
+    L.orc_gen_poisson_3d_csr.argtypes = [C.c_int64] * 3 + [C.c_void_p] * 3
+    L.orc_gen_poisson_3d_csr.restype = C.c_int64
+    L.orc_gen_convdiff_3d_csr.argtypes = [C.c_int64] * 3 + [C.c_double] * 4 + \
+        [C.c_void_p] * 3
+    L.orc_gen_convdiff_3d_csr.restype = C.c_int64
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__(f"{what}: oracle status {status}")
+        self.status = status
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Csr:
+    """Host CSR with int64 indices, the reference's layout
+    (Compressed_row_sparsity.hpp:97-107, Compressed_row_matrix.hpp:73-76)."""
+    nrows: int
+    ncols: int
+    row_ptr: np.ndarray
+    col_ind: np.ndarray
+    vals: np.ndarray
+
+    @property
+    def nnz(self):
+        return int(self.row_ptr[-1])
+
+    def at(self, r, c):
+        return lib().orc_csr_at(self.row_ptr, self.col_ind,
+                                np.ascontiguousarray(self.vals, np.float64),
+                                r, c)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.vals, self.col_ind, self.row_ptr),
+                             shape=(self.nrows, self.ncols))
+
+    def _struct(self, vals=None):
+        v = self.vals if vals is None else vals
+        return CsrD(self.nrows, self.ncols, _ptr(self.row_ptr),
+                    _ptr(self.col_ind), _ptr(v))
+
+
+def csr_pattern(nrows, ncols, rows, cols):
+    """Compressed_row_sparsity(Shape, indices):
+    Compressed_row_sparsity_Impl.cpp:10-40."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    rp = np.zeros(nrows + 1 if nrows > 0 else 1, np.int64)
+    ci = np.zeros(max(len(rows), 1), np.int64)
+    nnz = C.c_int64(0)
+    st = lib().orc_csr_pattern(nrows, ncols, len(rows), rows, cols, rp, ci,
+                               C.byref(nnz))
+    if st:
+        raise OracleError(st, "csr_pattern")
+    return rp, ci[:nnz.value].copy()
+
+
+def make_matrix(nrows, ncols, rows, cols, vals):
+    """make_matrix(shape, entries): matgen.hpp:25-52."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    vals = np.ascontiguousarray(vals, np.float64)
+    rp = np.zeros(nrows + 1 if nrows > 0 else 1, np.int64)
+    ci = np.zeros(max(len(rows), 1), np.int64)
+    v = np.zeros(max(len(rows), 1), np.float64)
+    nnz = C.c_int64(0)
+    st = lib().orc_make_matrix(nrows, ncols, len(rows), rows, cols, vals, rp,
+                               ci, v, C.byref(nnz))
+    if st:
+        raise OracleError(st, "make_matrix")
+    k = nnz.value
+    return Csr(nrows, ncols, rp, ci[:k].copy(), v[:k].copy())
+
+
+def spmv(A: Csr, x):
+    """multiply(A, x): sparse_blas.hpp:17-34 (dtype of x selects fp64/fp32)."""
+    if np.asarray(x).dtype == np.float32:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(A.nrows, np.float32)
+        lib().orc_spmv_f(A.nrows, A.row_ptr, A.col_ind,
+                         np.ascontiguousarray(A.vals, np.float32), x, y)
+        return y
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(A.nrows, np.float64)
+    lib().orc_spmv_d(A.nrows, A.row_ptr, A.col_ind, A.vals, x, y)
+    return y
+
+
+def extract_diagonal(A: Csr):
+    """unary.hpp:18-39."""
+    d = np.zeros(min(A.nrows, A.ncols), np.float64)
+    lib().orc_extract_diagonal(A.nrows, A.ncols, A.row_ptr, A.col_ind, A.vals,
+                               d)
+    return d
+
+
+def jacobi(A: Csr):
+    """jacobi.hpp:25-36; raises on an exact zero diagonal."""
+    d = np.zeros(min(A.nrows, A.ncols), np.float64)
+    st = lib().orc_jacobi(A.nrows, A.ncols, A.row_ptr, A.col_ind, A.vals, d)
+    if st:
+        raise OracleError(st, "jacobi")
+    return d
+
+
+def jacobi_apply(inv_d, r):
+    """jacobi.hpp:42-50."""
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.zeros_like(r)
+    lib().orc_jacobi_apply_d(len(r), np.ascontiguousarray(inv_d, np.float64),
+                             r, z)
+    return z
+
+
+def ilu0(A: Csr, modified: bool = False):
+    """ilu0 / milu0: ilu.hpp:35-168.  Returns (L, U)."""
+    n = A.nrows
+    cap = A.nnz + n + 1
+    lrp = np.zeros(n + 1, np.int64)
+    urp = np.zeros(n + 1, np.int64)
+    lci = np.zeros(cap, np.int64)
+    uci = np.zeros(cap, np.int64)
+    lv = np.zeros(cap, np.float64)
+    uv = np.zeros(cap, np.float64)
+    st = lib().orc_ilu0(A.nrows, A.ncols, A.row_ptr, A.col_ind, A.vals,
+                        1 if modified else 0, lrp, lci, lv, urp, uci, uv)
+    if st:
+        raise OracleError(st, "ilu0")
+    nl, nu = int(lrp[-1]), int(urp[-1])
+    return (Csr(n, n, lrp, lci[:nl].copy(), lv[:nl].copy()),
+            Csr(n, n, urp, uci[:nu].copy(), uv[:nu].copy()))
+
+
+def forward_solve(L: Csr, b):
+    """triangular_solve.hpp:27-55."""
+    b = np.ascontiguousarray(b)
+    x = np.zeros_like(b)
+    if b.dtype == np.float32:
+        lib().orc_forward_solve_f(L.nrows, L.row_ptr, L.col_ind,
+                                  L.vals.astype(np.float32), b, x)
+    else:
+        lib().orc_forward_solve_d(L.nrows, L.row_ptr, L.col_ind, L.vals,
+                                  b.astype(np.float64), x)
+    return x
+
+
+def backward_solve(U: Csr, b):
+    """triangular_solve.hpp:67-98."""
+    b = np.ascontiguousarray(b)
+    x = np.zeros_like(b)
+    if b.dtype == np.float32:
+        lib().orc_backward_solve_f(U.nrows, U.row_ptr, U.col_ind,
+                                   U.vals.astype(np.float32), b, x)
+    else:
+        lib().orc_backward_solve_d(U.nrows, U.row_ptr, U.col_ind, U.vals,
+                                   b.astype(np.float64), x)
+    return x
+
+
+def true_relres(A: Csr, b, x):
+    return lib().orc_true_relres_d(A.nrows, A.row_ptr, A.col_ind, A.vals,
+                                   np.ascontiguousarray(b, np.float64),
+                                   np.ascontiguousarray(x, np.float64))
+
+
+# ---------------------------------------------------------------- solvers
+
+@dataclass
+class Summary:
+    """CGSummary / Gmres_summary / Fgmres_summary / Bicgstab_summary
+    (conjugate_gradient.hpp:26-40 and siblings) plus a status code."""
+    residual_norm: float
+    computed_iterations: int
+    converged: bool
+    status: int
+    iteration_residuals: np.ndarray
+
+
+class _Op:
+    """Binds one preconditioner/operator slot to a C callback + context."""
+
+    def __init__(self, spec, A: Csr, dtype):
+        L = lib()
+        sfx = "f" if dtype == np.float32 else "d"
+        self.keep = []
+        if spec is None or spec == "identity":
+            self.fn = C.cast(getattr(L, f"orc_apply_identity_{sfx}"),
+                             C.c_void_p)
+            self.ctx = None
+            return
+        kind, data = spec if isinstance(spec, tuple) else (spec, None)
+        if kind == "csr":
+            M = data if data is not None else A
+            v = np.ascontiguousarray(M.vals, dtype)
+            s = M._struct(v)
+            self.keep += [v, s, M]
+            self.fn = C.cast(getattr(L, f"orc_apply_csr_{sfx}"), C.c_void_p)
+            self.ctx = C.cast(C.pointer(s), C.c_void_p)
+        elif kind == "jacobi":
+            inv_d = jacobi(A) if data is None else data
+            inv_d = np.ascontiguousarray(inv_d, dtype)
+            self.keep.append(inv_d)
+            self.fn = C.cast(getattr(L, f"orc_apply_jacobi_{sfx}"),
+                             C.c_void_p)
+            self.ctx = _ptr(inv_d)
+        elif kind in ("ilu", "milu"):
+            Lf, Uf = data if data is not None else ilu0(A, kind == "milu")
+            lv = np.ascontiguousarray(Lf.vals, dtype)
+            uv = np.ascontiguousarray(Uf.vals, dtype)
+            tmp = np.zeros(A.nrows, dtype)
+            s = IluD(Lf._struct(lv), Uf._struct(uv), _ptr(tmp))
+            self.keep += [Lf, Uf, lv, uv, tmp, s]
+            self.fn = C.cast(getattr(L, f"orc_apply_ilu_{sfx}"), C.c_void_p)
+            self.ctx = C.cast(C.pointer(s), C.c_void_p)
+        else:
+            raise ValueError(f"unknown operator kind {kind!r}")
+
+
+def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
+           dtype):
+    L = lib()
+    dtype = np.dtype(dtype)
+    sfx = "f" if dtype == np.float32 else "d"
+    n = A.nrows
+    b = np.ascontiguousarray(b, dtype)
+    x = np.zeros(n, dtype) if x0 is None else np.array(x0, dtype=dtype)
+    cfg = (ConfigF if sfx == "f" else ConfigD)(tol, restart, max_iterations)
+    summ = (SummaryF if sfx == "f" else SummaryD)()
+    hist = np.zeros(max(max_iterations, 1), dtype)
+    a_op = _Op(("csr", A), A, dtype)
+    ml = _Op(left, A, dtype)
+    if name == "fgmres":
+        getattr(L, f"orc_fgmres_{sfx}")(n, b, x, C.byref(cfg), a_op.fn,
+                                        a_op.ctx, ml.fn, ml.ctx,
+                                        C.byref(summ), _ptr(hist))
+    else:
+        mr = _Op(right, A, dtype)
+        getattr(L, f"orc_{name}_{sfx}")(n, b, x, C.byref(cfg), a_op.fn,
+                                        a_op.ctx, ml.fn, ml.ctx, mr.fn,
+                                        mr.ctx, C.byref(summ), _ptr(hist))
+    s = Summary(float(summ.residual_norm), int(summ.computed_iterations),
+                bool(summ.converged), int(summ.status),
+                hist[:summ.history_len].copy())
+    return x, s
+
+
+def cg(A, b, x0=None, *, tol, restart_iterations, max_iterations, left=None,
+       right=None, dtype=np.float64):
+    """conjugate_gradient(): conjugate_gradient.hpp:122-366."""
+    return _solve("cg", A, b, x0, tol, restart_iterations, max_iterations,
+                  left, right, dtype)
+
+
+def gmres(A, b, x0=None, *, tol, restart_dimension, max_iterations, left=None,
+          right=None, dtype=np.float64):
+    """gmres(): gmres.hpp:99-427."""
+    return _solve("gmres", A, b, x0, tol, restart_dimension, max_iterations,
+                  left, right, dtype)
+
+
+def fgmres(A, b, x0=None, *, tol, restart_dimension, max_iterations,
+           preconditioner=None, dtype=np.float64):
+    """fgmres(): fgmres.hpp:96-470."""
+    return _solve("fgmres", A, b, x0, tol, restart_dimension, max_iterations,
+                  preconditioner, None, dtype)
+
+
+def bicgstab(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
+             dtype=np.float64):
+    """bicgstab(): bicgstab.hpp:86-410."""
+    return _solve("bicgstab", A, b, x0, tol, 0, max_iterations, left, right,
+                  dtype)
+
+
+# ------------------------------------------------------------- generators
+
+def _entries(fn, *args):
+    L = lib()
+    m = getattr(L, fn)(*args, None, None, None)
+    r = np.zeros(m, np.int64)
+    c = np.zeros(m, np.int64)
+    v = np.zeros(m, np.float64)
+    getattr(L, fn)(*args, _ptr(r), _ptr(c), _ptr(v))
+    return r, c, v
+
+
+def tridiagonal_matrix(n, sub, diag, sup):
+    """matgen.hpp:104-117."""
+    r, c, v = _entries("orc_gen_tridiagonal", n, sub, diag, sup)
+    return make_matrix(n, n, r, c, v)
+
+
+def poisson_2d(nx, ny=None, shift=0.0):
+    """poisson_2d(nx, ny[, shift]): matgen.hpp:173-265."""
+    ny = nx if ny is None else ny
+    r, c, v = _entries("orc_gen_poisson_2d", nx, ny, shift)
+    return make_matrix(nx * ny, nx * ny, r, c, v)
+
+
+def convection_diffusion_2d(nx, ny, d, cx, cy):
+    """matgen.hpp:274-325 (upwind)."""
+    r, c, v = _entries("orc_gen_convection_diffusion_2d", nx, ny, d, cx, cy)
+    return make_matrix(nx * ny, nx * ny, r, c, v)
+
+
+def convdiff_centered_2d(nx, ny, d, cx, cy, shift=0.0):
+    """matgen.hpp:339-386."""
+    r, c, v = _entries("orc_gen_convdiff_centered_2d", nx, ny, d, cx, cy,
+                       shift)
+    return make_matrix(nx * ny, nx * ny, r, c, v)
+
+
+def nonsymmetric_sample():
+    """matgen.hpp:147-164."""
+    r, c, v = _entries("orc_gen_nonsymmetric_sample")
+    return make_matrix(4, 4, r, c, v)
+
+
+def _csr_gen(fn, n, *args):
+    L = lib()
+    nnz = getattr(L, fn)(*args, None, None, None)
+    rp = np.zeros(n + 1, np.int64)
+    ci = np.zeros(nnz, np.int64)
+    v = np.zeros(nnz, np.float64)
+    getattr(L, fn)(*args, _ptr(rp), _ptr(ci), _ptr(v))
+    return Csr(n, n, rp, ci, v)
+
+
+def poisson_3d(nx, ny=None, nz=None):
+    """7-point Dirichlet Laplacian (no reference counterpart)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _csr_gen("orc_gen_poisson_3d_csr", nx * ny * nz, nx, ny, nz)
+
+
+def convdiff_3d(nx, ny, nz, d, cx, cy, cz):
+    """7-point upwind convection-diffusion, Dirichlet diagonal (no reference
+    counterpart; extends matgen.hpp:292-321 to 3D)."""
+    return _csr_gen("orc_gen_convdiff_3d_csr", nx * ny * nz, nx, ny, nz, d,
+                    cx, cy, cz)
+
+
+def splitmix64_at(seed, k):
+    return int(lib().orc_splitmix64_at(seed, k))
+
+
+def uniform_pm1(seed, n):
+    out = np.zeros(n, np.float64)
+    lib().orc_uniform_pm1(seed, n, out)
+    return out

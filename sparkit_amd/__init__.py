@@ -1,0 +1,440 @@
+"""sparkit_amd -- MI355X (gfx950) implementation of sparkit's CSR SpMV +
+preconditioned Krylov solve path.
+
+Python mirror of the reference's C++ interface (namespace
+sparkit::data::detail), over the C ABI in include/sparkit_hip.h:
+
+  CsrMatrix              Compressed_row_matrix<T> held in HBM
+                         (Compressed_row_matrix.hpp:20-132)
+  multiply(A, x)         sparse_blas.hpp:17-34
+  extract_diagonal(A)    unary.hpp:18-39
+  jacobi(A), jacobi_apply                jacobi.hpp:25-50
+  ilu0(A), milu0(A), ilu_apply           ilu.hpp:35-179
+  conjugate_gradient(b, x, cfg, A, M_L, M_R)   conjugate_gradient.hpp:339-366
+  gmres(b, x, cfg, A, M_L, M_R)                gmres.hpp:400-427
+  fgmres(b, x, cfg, A, M)                      fgmres.hpp:452-470
+  bicgstab(b, x, cfg, A, M_L, M_R)             bicgstab.hpp:383-410
+
+Operators and preconditioners are device objects; vectors are
+DeviceVector (HBM).  Everything runs on the HIP library; there is no CPU
+path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (F32, F64, BREAKDOWN, InvalidArgument, LogicError, SpkError,
+                   call, load)
+
+__all__ = [
+    "init", "synchronize", "DeviceVector", "CsrMatrix", "Preconditioner",
+    "identity", "jacobi", "ilu0", "milu0", "spai", "multiply",
+    "extract_diagonal", "jacobi_apply", "ilu_apply", "CGConfig",
+    "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
+    "conjugate_gradient", "gmres", "fgmres", "bicgstab", "true_relres",
+    "F32", "F64", "SpkError", "InvalidArgument", "LogicError", "Timer",
+    "poisson_2d", "poisson_3d", "convdiff_3d", "uniform_pm1",
+]
+
+_DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
+
+
+def _code(dtype) -> int:
+    return _DT[np.dtype(dtype)]
+
+
+def init(device: int = 0):
+    call("spk_init", device)
+
+
+def synchronize():
+    call("spk_synchronize")
+
+
+def device_info():
+    cus = C.c_int(0)
+    buf = C.create_string_buffer(256)
+    call("spk_device_info", C.byref(cus), buf, 256)
+    return cus.value, buf.value.decode()
+
+
+class Timer:
+    """HIP events on the library stream (the stream every kernel uses)."""
+
+    def start(self):
+        call("spk_timer_start")
+
+    def stop(self) -> float:
+        ms = C.c_float(0)
+        call("spk_timer_stop", C.byref(ms))
+        return ms.value
+
+
+class DeviceVector:
+    """A vector in HBM (float64 or float32)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        call("spk_malloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or np.asarray(a).dtype)
+        v = cls(a.size, a.dtype)
+        if a.size:
+            call("spk_memcpy_htod", v.ptr, a.ctypes.data_as(C.c_void_p),
+                 v.nbytes)
+        return v
+
+    @classmethod
+    def zeros(cls, n, dtype=np.float64):
+        v = cls(n, dtype)
+        call("spk_memset", v.ptr, 0, max(v.nbytes, 1))
+        return v
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            call("spk_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr,
+                 self.nbytes)
+        return out
+
+    def copy_from(self, other: "DeviceVector"):
+        assert other.n == self.n and other.dtype == self.dtype
+        call("spk_memcpy_dtod", self.ptr, other.ptr, self.nbytes)
+
+    def astype(self, dtype):
+        out = DeviceVector(self.n, dtype)
+        call("spk_convert", self.n, _code(self.dtype), self.ptr,
+             _code(out.dtype), out.ptr)
+        return out
+
+    def __len__(self):
+        return self.n
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib._lib is not None:
+                _lib._lib.spk_free(self.ptr)
+        except Exception:
+            pass
+
+
+class CsrMatrix:
+    """Compressed_row_matrix<double> resident in HBM: int32 row_ptr/col_ind
+    (narrowed from the reference's int64) and double values, plus the
+    row-binned SpMV plan."""
+
+    def __init__(self, handle):
+        self.h = handle
+        r, c, z = C.c_int64(), C.c_int64(), C.c_int64()
+        call("spk_csr_shape", self.h, C.byref(r), C.byref(c), C.byref(z))
+        self.nrows, self.ncols, self.nnz = r.value, c.value, z.value
+
+    @classmethod
+    def from_host(cls, nrows, ncols, row_ptr, col_ind, vals):
+        rp = np.ascontiguousarray(row_ptr, np.int64)
+        ci = np.ascontiguousarray(col_ind, np.int64)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = C.c_void_p()
+        call("spk_csr_create", int(nrows), int(ncols), int(len(ci)),
+             rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+             v.ctypes.data_as(C.c_void_p), C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_oracle(cls, A):
+        """From any object with nrows/ncols/row_ptr/col_ind/vals."""
+        return cls.from_host(A.nrows, A.ncols, A.row_ptr, A.col_ind, A.vals)
+
+    def download(self):
+        rp = np.zeros(self.nrows + 1, np.int64)
+        ci = np.zeros(max(self.nnz, 1), np.int64)
+        v = np.zeros(max(self.nnz, 1), np.float64)
+        call("spk_csr_download", self.h, rp.ctypes.data_as(C.c_void_p),
+             ci.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p))
+        return rp, ci[:self.nnz], v[:self.nnz]
+
+    def plan_info(self):
+        nb, nl = C.c_int64(), C.c_int64()
+        call("spk_csr_plan_info", self.h, C.byref(nb), C.byref(nl))
+        return nb.value, nl.value
+
+    def spmv(self, x: DeviceVector, y: DeviceVector):
+        call("spk_csr_spmv", self.h, _code(x.dtype), x.ptr, y.ptr)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib._lib is not None:
+                _lib._lib.spk_csr_destroy(self.h)
+        except Exception:
+            pass
+
+
+def multiply(A: CsrMatrix, x: DeviceVector) -> DeviceVector:
+    """multiply(A, x) (sparse_blas.hpp:17-34): returns a new vector."""
+    y = DeviceVector(A.nrows, x.dtype)
+    A.spmv(x, y)
+    return y
+
+
+def extract_diagonal(A: CsrMatrix) -> DeviceVector:
+    d = DeviceVector(min(A.nrows, A.ncols), np.float64)
+    call("spk_csr_extract_diagonal", A.h, d.ptr)
+    return d
+
+
+class Preconditioner:
+    def __init__(self, handle, kind, n, owner=None):
+        self.h = handle
+        self.kind = kind
+        self.n = n
+        self._owner = owner  # keeps an SpMV preconditioner's store alive
+
+    def apply(self, r: DeviceVector, z: DeviceVector):
+        call("spk_precond_apply", self.h, _code(r.dtype), r.ptr, z.ptr)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib._lib is not None:
+                _lib._lib.spk_precond_destroy(self.h)
+        except Exception:
+            pass
+
+
+class _Identity:
+    """The identity slot (conjugate_gradient_test.cpp:45-47)."""
+    h = None
+    kind = "identity"
+
+
+identity = _Identity()
+
+
+def jacobi(A: CsrMatrix) -> Preconditioner:
+    """jacobi(A) (jacobi.hpp:25-36); raises InvalidArgument on a zero
+    diagonal entry."""
+    h = C.c_void_p()
+    call("spk_precond_create_jacobi", A.h, C.byref(h))
+    return Preconditioner(h, "jacobi", min(A.nrows, A.ncols))
+
+
+def jacobi_apply(M: Preconditioner, r: DeviceVector, z: DeviceVector):
+    """jacobi_apply (jacobi.hpp:42-50)."""
+    M.apply(r, z)
+
+
+class Ilu0(Preconditioner):
+    def factors(self):
+        """(L, U) in the reference layout: ((rp, ci, v), (rp, ci, v))."""
+        nl, nu = C.c_int64(), C.c_int64()
+        call("spk_ilu0_factor_nnz", self.h, C.byref(nl), C.byref(nu))
+        n = self.n
+        lrp = np.zeros(n + 1, np.int64)
+        urp = np.zeros(n + 1, np.int64)
+        lci = np.zeros(nl.value, np.int64)
+        uci = np.zeros(nu.value, np.int64)
+        lv = np.zeros(nl.value, np.float64)
+        uv = np.zeros(nu.value, np.float64)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        call("spk_ilu0_download", self.h, P(lrp), P(lci), P(lv), P(urp),
+             P(uci), P(uv))
+        return (lrp, lci, lv), (urp, uci, uv)
+
+    def levels(self):
+        a, b = C.c_int64(), C.c_int64()
+        call("spk_ilu0_levels", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def forward(self, b: DeviceVector, x: DeviceVector):
+        call("spk_ilu0_forward", self.h, _code(b.dtype), b.ptr, x.ptr)
+
+    def backward(self, b: DeviceVector, x: DeviceVector):
+        call("spk_ilu0_backward", self.h, _code(b.dtype), b.ptr, x.ptr)
+
+
+def ilu0(A: CsrMatrix) -> Ilu0:
+    """ilu0(A) (ilu.hpp:155-159)."""
+    h = C.c_void_p()
+    call("spk_precond_create_ilu0", A.h, 0, C.byref(h))
+    return Ilu0(h, "ilu0", A.nrows)
+
+
+def milu0(A: CsrMatrix) -> Ilu0:
+    """milu0(A) (ilu.hpp:164-168)."""
+    h = C.c_void_p()
+    call("spk_precond_create_ilu0", A.h, 1, C.byref(h))
+    return Ilu0(h, "milu0", A.nrows)
+
+
+def ilu_apply(F: Ilu0, r: DeviceVector, z: DeviceVector):
+    """ilu_apply (ilu.hpp:173-179)."""
+    F.apply(r, z)
+
+
+def spai(M: CsrMatrix) -> Preconditioner:
+    """z = M r for an explicit sparse approximate inverse M."""
+    h = C.c_void_p()
+    call("spk_precond_create_spmv", M.h, C.byref(h))
+    return Preconditioner(h, "spai", M.nrows, owner=M)
+
+
+# ---------------------------------------------------------------- solvers
+
+@dataclass
+class CGConfig:
+    """CGConfig (conjugate_gradient.hpp:47-74) + the arithmetic precision of
+    the Krylov vectors."""
+    tolerance: float = 0.0
+    restart_iterations: int = 0
+    max_iterations: int = 0
+    collect_residuals: bool = False
+    precision: int = F64
+
+
+@dataclass
+class GmresConfig:
+    """Gmres_config (gmres.hpp:24-47)."""
+    tolerance: float = 0.0
+    restart_dimension: int = 0
+    max_iterations: int = 0
+    collect_residuals: bool = False
+    precision: int = F64
+
+
+FgmresConfig = GmresConfig  # Fgmres_config has the same fields
+
+
+@dataclass
+class BicgstabConfig:
+    """Bicgstab_config (bicgstab.hpp:24-42)."""
+    tolerance: float = 0.0
+    max_iterations: int = 0
+    collect_residuals: bool = False
+    precision: int = F64
+
+
+@dataclass
+class Summary:
+    """CGSummary / Gmres_summary / Fgmres_summary / Bicgstab_summary."""
+    residual_norm: float = 0.0
+    computed_iterations: int = 0
+    converged: bool = False
+    iteration_residuals: list = field(default_factory=list)
+    breakdown: bool = False
+
+
+def _cfg(c, restart):
+    return _lib.SolverConfig(float(c.tolerance), int(restart),
+                             int(c.max_iterations),
+                             1 if c.collect_residuals else 0,
+                             int(c.precision))
+
+
+def _run(name, args, cfg):
+    s = _lib.Summary()
+    hist = np.zeros(max(cfg.max_iterations + 1, 1), np.float64)
+    call(name, *args, C.byref(s), hist.ctypes.data_as(C.c_void_p))
+    return Summary(s.residual_norm, s.computed_iterations, bool(s.converged),
+                   hist[:s.history_len].tolist(), s.status == BREAKDOWN)
+
+
+def _h(M):
+    return None if M is None else M.h
+
+
+def _vec_check(A, b, x, precision):
+    dt = np.float32 if precision == F32 else np.float64
+    if b.dtype != dt or x.dtype != dt:
+        raise InvalidArgument(1, "solver", "b and x must match the precision")
+    if b.n != A.nrows or x.n != A.nrows:
+        raise InvalidArgument(1, "solver", "vector size mismatch")
+
+
+def conjugate_gradient(b, x, cfg: CGConfig, A: CsrMatrix, M_left=identity,
+                       M_right=identity) -> Summary:
+    """conjugate_gradient() (conjugate_gradient.hpp:339-366); x updated in
+    place."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, cfg.restart_iterations)
+    return _run("spk_cg_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
+                                 b.ptr, x.ptr), cfg)
+
+
+def gmres(b, x, cfg: GmresConfig, A: CsrMatrix, M_left=identity,
+          M_right=identity) -> Summary:
+    """gmres() (gmres.hpp:400-427)."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, cfg.restart_dimension)
+    return _run("spk_gmres_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
+                                    b.ptr, x.ptr), cfg)
+
+
+def fgmres(b, x, cfg: GmresConfig, A: CsrMatrix, M=identity) -> Summary:
+    """fgmres() (fgmres.hpp:452-470)."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, cfg.restart_dimension)
+    return _run("spk_fgmres_solve", (A.h, _h(M), C.byref(c), b.ptr, x.ptr),
+                cfg)
+
+
+def bicgstab(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
+             M_right=identity) -> Summary:
+    """bicgstab() (bicgstab.hpp:383-410); computed_iterations counts
+    matrix-vector products."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, 0)
+    return _run("spk_bicgstab_solve", (A.h, _h(M_left), _h(M_right),
+                                       C.byref(c), b.ptr, x.ptr), cfg)
+
+
+def true_relres(A: CsrMatrix, b: DeviceVector, x: DeviceVector) -> float:
+    """||b - A x|| / ||b|| in FP64 on the device."""
+    r = C.c_double()
+    call("spk_true_relres", A.h, b.ptr, x.ptr, C.byref(r))
+    return r.value
+
+
+# ------------------------------------------------------------- generators
+
+def _gen(name, *args):
+    h = C.c_void_p()
+    call(name, *args, C.byref(h))
+    return CsrMatrix(h)
+
+
+def poisson_2d(nx, ny=None, shift=0.0) -> CsrMatrix:
+    """poisson_2d (matgen.hpp:173-265) built on the device."""
+    return _gen("spk_gen_poisson_2d", nx, nx if ny is None else ny,
+                float(shift))
+
+
+def poisson_3d(nx, ny=None, nz=None) -> CsrMatrix:
+    """7-point Dirichlet Laplacian (diagonal 6) built on the device."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _gen("spk_gen_poisson_3d", nx, ny, nz)
+
+
+def convdiff_3d(nx, ny, nz, d, cx, cy, cz) -> CsrMatrix:
+    """7-point upwind convection-diffusion, Dirichlet diagonal."""
+    return _gen("spk_gen_convdiff_3d", nx, ny, nz, float(d), float(cx),
+                float(cy), float(cz))
+
+
+def uniform_pm1(seed: int, n: int) -> DeviceVector:
+    """splitmix64 uniform[-1,1] vector generated on the device."""
+    v = DeviceVector(n, np.float64)
+    call("spk_fill_uniform_pm1", seed, n, v.ptr)
+    return v

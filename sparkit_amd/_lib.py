@@ -1,0 +1,156 @@
+"""Loader and ctypes declarations of libsparkit_hip.so (the C ABI declared in
+include/sparkit_hip.h).  There is no fallback: if the library is missing or
+cannot be loaded, every entry point raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsparkit_hip.so")
+
+# status codes (include/sparkit_hip.h)
+OK = 0
+INVALID_ARGUMENT = 1
+LOGIC_ERROR = 2
+OUT_OF_MEMORY = 3
+HIP_ERROR = 4
+BREAKDOWN = 5
+NO_DEVICE = 6
+
+F64 = 0
+F32 = 1
+
+
+class SpkError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: status {status}: {msg}")
+        self.status = status
+
+
+class InvalidArgument(SpkError, ValueError):
+    """std::invalid_argument in the reference."""
+
+
+class LogicError(SpkError):
+    """std::logic_error in the reference."""
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("restart", C.c_int64),
+                ("max_iterations", C.c_int64),
+                ("collect_residuals", C.c_int32), ("precision", C.c_int32)]
+
+
+class Summary(C.Structure):
+    _fields_ = [("residual_norm", C.c_double),
+                ("computed_iterations", C.c_int64),
+                ("converged", C.c_int32), ("status", C.c_int32),
+                ("history_len", C.c_int64)]
+
+
+vp = C.c_void_p
+i64 = C.c_int64
+u64 = C.c_uint64
+dbl = C.c_double
+i32 = C.c_int
+P_i64 = C.POINTER(C.c_int64)
+P_dbl = C.POINTER(C.c_double)
+
+# name: (argtypes, restype)  -- restype None means spk_status (int)
+_SIGS = {
+    "spk_last_error": ([], C.c_char_p),
+    "spk_init": ([i32], None),
+    "spk_finalize": ([], None),
+    "spk_set_stream": ([vp], None),
+    "spk_get_stream": ([], vp),
+    "spk_synchronize": ([], None),
+    "spk_device_info": ([C.POINTER(C.c_int), C.c_char_p, C.c_size_t], None),
+    "spk_malloc": ([C.POINTER(vp), C.c_size_t], None),
+    "spk_free": ([vp], None),
+    "spk_memcpy_htod": ([vp, vp, C.c_size_t], None),
+    "spk_memcpy_dtoh": ([vp, vp, C.c_size_t], None),
+    "spk_memcpy_dtod": ([vp, vp, C.c_size_t], None),
+    "spk_memset": ([vp, i32, C.c_size_t], None),
+    "spk_timer_start": ([], None),
+    "spk_timer_stop": ([C.POINTER(C.c_float)], None),
+    "spk_csr_create": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)], None),
+    "spk_csr_create_device": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)],
+                              None),
+    "spk_csr_destroy": ([vp], None),
+    "spk_csr_shape": ([vp, P_i64, P_i64, P_i64], None),
+    "spk_csr_download": ([vp, vp, vp, vp], None),
+    "spk_csr_device_arrays": ([vp, i32, C.POINTER(vp), C.POINTER(vp),
+                               C.POINTER(vp)], None),
+    "spk_csr_plan_info": ([vp, P_i64, P_i64], None),
+    "spk_csr_spmv": ([vp, i32, vp, vp], None),
+    "spk_csr_extract_diagonal": ([vp, vp], None),
+    "spk_dot": ([i32, i64, vp, vp, P_dbl], None),
+    "spk_axpby": ([i32, i64, dbl, vp, dbl, vp], None),
+    "spk_convert": ([i64, i32, vp, i32, vp], None),
+    "spk_precond_create_jacobi": ([vp, C.POINTER(vp)], None),
+    "spk_precond_create_ilu0": ([vp, i32, C.POINTER(vp)], None),
+    "spk_precond_create_identity": ([i64, C.POINTER(vp)], None),
+    "spk_precond_create_spmv": ([vp, C.POINTER(vp)], None),
+    "spk_precond_destroy": ([vp], None),
+    "spk_precond_apply": ([vp, i32, vp, vp], None),
+    "spk_ilu0_factor_nnz": ([vp, P_i64, P_i64], None),
+    "spk_ilu0_download": ([vp, vp, vp, vp, vp, vp, vp], None),
+    "spk_ilu0_levels": ([vp, P_i64, P_i64], None),
+    "spk_ilu0_forward": ([vp, i32, vp, vp], None),
+    "spk_ilu0_backward": ([vp, i32, vp, vp], None),
+    "spk_cg_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                      C.POINTER(Summary), vp], None),
+    "spk_gmres_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                         C.POINTER(Summary), vp], None),
+    "spk_fgmres_solve": ([vp, vp, C.POINTER(SolverConfig), vp, vp,
+                          C.POINTER(Summary), vp], None),
+    "spk_bicgstab_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                            C.POINTER(Summary), vp], None),
+    "spk_true_relres": ([vp, vp, vp, P_dbl], None),
+    "spk_fill_uniform_pm1": ([u64, i64, vp], None),
+    "spk_gen_poisson_2d": ([i64, i64, dbl, C.POINTER(vp)], None),
+    "spk_gen_poisson_3d": ([i64, i64, i64, C.POINTER(vp)], None),
+    "spk_gen_convdiff_3d": ([i64, i64, i64, dbl, dbl, dbl, dbl,
+                             C.POINTER(vp)], None),
+}
+
+_lib = None
+
+
+def exported_names():
+    return sorted(_SIGS)
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is not built: run __graft_entry__.build() (or make -C "
+            "sparkit_amd/csrc); the HIP path has no fallback")
+    L = C.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int if res is None else res
+    _lib = L
+    return L
+
+
+def check(status: int, where: str):
+    if status == OK:
+        return
+    msg = load().spk_last_error().decode(errors="replace")
+    if status == INVALID_ARGUMENT:
+        raise InvalidArgument(status, where, msg)
+    if status == LOGIC_ERROR:
+        raise LogicError(status, where, msg)
+    raise SpkError(status, where, msg)
+
+
+def call(name: str, *args):
+    st = getattr(load(), name)(*args)
+    check(st, name)

@@ -1,0 +1,220 @@
+// spk_bicgstab.hip -- device BiCGSTAB (bicgstab.hpp:86-361): van der Vorst's
+// recurrence with both preconditioner slots composed on p and s, the
+// half-step convergence test on |s|, and computed_iterations counting
+// matrix-vector products.  Scalars stay on the device; the host queues
+// steps in chunks and polls the done flag.
+//
+// Breakdowns the reference leaves undetected stop the solve with status
+// SPK_BREAKDOWN: rho_new == 0 (the reference returns without consuming
+// budget and loops forever, bicgstab.hpp:203-204) and omega == 0 (it would
+// divide by omega in the next beta, :206, :222-223).
+#include "spk_solver_common.hpp"
+
+namespace spk {
+namespace {
+
+struct BiState {
+  double bnorm, tol, resnorm;
+  double rho, rho_new, alpha, omega, beta;
+  int64_t iters;
+  int64_t half_step;  // step index whose half-step test converged, else -1
+  int32_t done, converged, breakdown, collect;
+  double* history;
+};
+
+template <typename T>
+struct EpiBiV {  // v = A p^, partial <r^, v>; last block: alpha
+  static constexpr bool kReduce = true;
+  const T* __restrict__ rh;
+  T* __restrict__ v;
+  BiState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    v[i] = s;
+    return (double)rh[i] * (double)s;
+  }
+  __device__ void finalize(double t) const {
+    st->iters += 1;
+    st->alpha = st->rho_new / t;
+  }
+};
+
+template <typename T>
+void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
+                    const spk_solver_config* cfg, const T* b, T* x,
+                    spk_summary* out, double* history) {
+  const int64_t n = A->nrows;
+  const size_t vb = (size_t)(n > 0 ? n : 1) * sizeof(T);
+  Workspace& ws = workspace();
+  T* r = (T*)ws.get(kWsBicgstab + 0, vb);
+  T* rh = (T*)ws.get(kWsBicgstab + 1, vb);
+  T* p = (T*)ws.get(kWsBicgstab + 2, vb);
+  T* v = (T*)ws.get(kWsBicgstab + 3, vb);
+  T* sv = (T*)ws.get(kWsBicgstab + 4, vb);
+  T* t = (T*)ws.get(kWsBicgstab + 5, vb);
+  T* ph = (T*)ws.get(kWsBicgstab + 6, vb);
+  T* sh = (T*)ws.get(kWsBicgstab + 7, vb);
+  T* tmp = (T*)ws.get(kWsBicgstab + 8, vb);
+  double* dhist = cfg->collect_residuals
+                      ? (double*)ws.get(kWsBicgstab + 9, sizeof(double) *
+                                        (size_t)(std::max<int64_t>(cfg->max_iterations, 1) + 1))
+                      : nullptr;
+  DevState<BiState> st(kWsBicgstab + 10);
+  BiState h{};
+  h.tol = cfg->tolerance;
+  h.collect = cfg->collect_residuals ? 1 : 0;
+  h.history = dhist;
+  h.half_step = -1;
+  h.rho = h.alpha = h.omega = 1.0;
+  st.upload(h);
+  BiState* S = st.d;
+  hipStream_t s = stream();
+  const int* done = &S->done;
+
+  launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+    acc[0] += (double)b[i] * (double)b[i];
+  }, [=] __device__(const double* tt) { S->bnorm = sqrt(tt[0]); });
+  h = st.download();
+  if (h.bnorm == 0.0) {  // bicgstab.hpp:160-165
+    launch_map(n, [=] __device__(int64_t i) { x[i] = T(0); });
+    *out = spk_summary{0.0, 0, 1, SPK_OK, 0};
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    return;
+  }
+  // r = b - A x; r^ = r; v = p = 0 (:167-174)
+  launch_spmv(*A, (const T*)x, EpiResidual<T>{b, r}, s);
+  launch_map(n, [=] __device__(int64_t i) {
+    rh[i] = r[i];
+    v[i] = T(0);
+    p[i] = T(0);
+  });
+
+  // M_R^-1 (M_L^-1 v) into out (the reference's tmp + second slot); with
+  // both slots the identity the input itself is used
+  auto compose = [&](const T* in, T* outv) -> const T* {
+    const bool l = !is_identity(ML), rr = !is_identity(MR);
+    if (!l && !rr) return in;
+    if (l && !rr) { ML->apply(in, outv); return outv; }
+    if (!l && rr) { MR->apply(in, outv); return outv; }
+    ML->apply(in, tmp);
+    MR->apply((const T*)tmp, outv);
+    return outv;
+  };
+  auto issue = [&](int64_t k) {
+    // rho_new = <r^, r>; beta (:203-206)
+    launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+      acc[0] += (double)rh[i] * (double)r[i];
+    }, [=] __device__(const double* tt) {
+      double rn = tt[0];
+      S->rho_new = rn;
+      if (rn == 0.0) {
+        S->breakdown = 1;
+        S->done = 1;
+        return;
+      }
+      S->beta = (rn / S->rho) * (S->alpha / S->omega);
+    }, done);
+    // p = r + beta (p - omega v) (:232-238)
+    launch_map(n, [=] __device__(int64_t i) {
+      p[i] = (T)((double)r[i] + S->beta * ((double)p[i] - S->omega * (double)v[i]));
+    }, done);
+    // p^ = M_R M_L p (:240-244); v = A p^ (:246-250); alpha (:211)
+    const T* php = compose(p, ph);
+    launch_spmv(*A, php, EpiBiV<T>{rh, v, S, done}, s);
+    // s = r - alpha v; half-step test on |s| (:252-278)
+    launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+      T si = (T)((double)r[i] - S->alpha * (double)v[i]);
+      sv[i] = si;
+      acc[0] += (double)si * (double)si;
+    }, [=] __device__(const double* tt) {
+      double sn = sqrt(tt[0]);
+      S->resnorm = sn;
+      if (S->collect) S->history[S->iters - 1] = sn;
+      if (sn / S->bnorm < S->tol) {
+        S->half_step = k;
+        S->converged = 1;
+        S->done = 1;
+      }
+    }, done);
+    // half-step exit: x += alpha p^ (:271-278); runs only for this step
+    launch_map(n, [=] __device__(int64_t i) {
+      if (S->half_step == k) x[i] = (T)((double)x[i] + S->alpha * (double)php[i]);
+    });
+    // s^ = M_R M_L s; t = A s^ (:280-290)
+    const T* shp = compose(sv, sh);
+    launch_spmv(*A, shp, EpiStore<T>{t, done}, s);
+    // omega = <t,s>/<t,t> (:292-297)
+    launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {
+      double ti = (double)t[i];
+      acc[0] += ti * ti;
+      acc[1] += ti * (double)sv[i];
+    }, [=] __device__(const double* tt) {
+      S->iters += 1;
+      double om = tt[0] == 0.0 ? 0.0 : tt[1] / tt[0];
+      S->omega = om;
+      if (om == 0.0) {
+        S->breakdown = 1;
+        S->done = 1;
+      }
+    }, done);
+    // x += alpha p^ + omega s^; r = s - omega t; |r| (:299-324)
+    launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+      x[i] = (T)((double)x[i] + (S->alpha * (double)php[i] + S->omega * (double)shp[i]));
+      T ri = (T)((double)sv[i] - S->omega * (double)t[i]);
+      r[i] = ri;
+      acc[0] += (double)ri * (double)ri;
+    }, [=] __device__(const double* tt) {
+      S->rho = S->rho_new;
+      double rn = sqrt(tt[0]);
+      S->resnorm = rn;
+      if (S->collect) S->history[S->iters - 1] = rn;
+      if (rn / S->bnorm < S->tol) {
+        S->converged = 1;
+        S->done = 1;
+      }
+    }, done);
+  };
+  // each step consumes two matvecs; steps run while budget remains (:176)
+  const int64_t steps = (cfg->max_iterations + 1) / 2;
+  auto is_done = [&]() { return st.download().done != 0; };
+  chunked_loop(steps, issue, is_done, 2, 32);
+
+  h = st.download();
+  out->residual_norm = h.resnorm;
+  out->computed_iterations = h.iters;
+  out->converged = h.converged;
+  out->status = h.breakdown ? SPK_BREAKDOWN : SPK_OK;
+  // one history entry per residual test: per matvec pair, the |s| test and
+  // (unless it converged) the |r| test
+  out->history_len = 0;
+  if (h.collect) out->history_len = h.iters;
+  if (history && h.collect && h.iters > 0) {
+    SPK_HIP_TRY(hipMemcpyAsync(history, dhist, h.iters * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" spk_status spk_bicgstab_solve(spk_csr A, spk_precond M_left,
+                                         spk_precond M_right,
+                                         const spk_solver_config* cfg,
+                                         const void* d_b, void* d_x,
+                                         spk_summary* summary,
+                                         double* history) {
+  return guarded([&] {
+    check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
+    SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
+    *summary = spk_summary{};
+    if (cfg->precision == SPK_F32)
+      bicgstab_solve<float>(A, M_left, M_right, cfg, (const float*)d_b,
+                            (float*)d_x, summary, history);
+    else
+      bicgstab_solve<double>(A, M_left, M_right, cfg, (const double*)d_b,
+                             (double*)d_x, summary, history);
+  });
+}

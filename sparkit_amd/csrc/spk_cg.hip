@@ -1,0 +1,232 @@
+// spk_cg.hip -- device conjugate gradient with the reference's unified
+// left/right preconditioned recurrence (conjugate_gradient.hpp:122-312):
+//
+//   every restart_iterations steps: r = b - A x; z = M_L r; rz = <r,z>; p = z
+//   p^ = M_R p; q = A p^; alpha = rz / <p,q>
+//   x += alpha p^; r -= alpha q; |r| / |b| < tol -> stop
+//   z = M_L r; beta = <r,z>_new / rz; p = z + beta p
+//
+// Scalars (rz, alpha, beta, |r|, the iteration count and the convergence
+// flag) live on the device and are updated by the one-block finisher that
+// follows each fused reduction kernel; the host only queues kernels and
+// polls the flag once per chunk.  With M_L = Jacobi and M_R = identity
+// (Jacobi-PCG) one iteration is three kernels plus two finishers:
+//   K1  q = A p, partial <p,q>;  finisher: alpha = rz/<p,q>
+//   K2  x += alpha p; r -= alpha q; <r,r>, <r, D^-1 r>;  finisher: |r|,
+//       convergence flag, beta, rz
+//   K3  p = D^-1 r + beta p
+// HBM traffic per iteration: B_spmv + 11 s n bytes (K2 reads x,p,r,q,D^-1
+// and writes x,r; K3 reads r,D^-1,p and writes p).
+#include "spk_solver_common.hpp"
+
+namespace spk {
+namespace {
+
+struct CgState {
+  double bnorm, rz, alpha, beta, resnorm, tol;
+  int64_t iters;
+  int32_t done, converged, collect, pad;
+  double* history;
+};
+
+template <typename T>
+struct EpiCgDot {  // q = A p^, partial <p, q>
+  static constexpr bool kReduce = true;
+  const T* __restrict__ p;
+  T* __restrict__ q;
+  CgState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    q[i] = s;
+    return (double)p[i] * (double)s;
+  }
+  __device__ void finalize(double pq) const { st->alpha = st->rz / pq; }
+};
+
+template <typename T>
+struct EpiCgRestartJacobi {  // r = b - A x; z = D^-1 r; p = z; <r, z>
+  static constexpr bool kReduce = true;
+  const T* __restrict__ b;
+  const T* __restrict__ invd;
+  T* __restrict__ r;
+  T* __restrict__ p;
+  CgState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    T ri = b[i] - s;
+    T zi = invd[i] * ri;
+    r[i] = ri;
+    p[i] = zi;
+    return (double)ri * (double)zi;
+  }
+  __device__ void finalize(double rz) const { st->rz = rz; }
+};
+
+template <typename T>
+struct EpiCgResidual {  // r = b - A x (generic restart)
+  static constexpr bool kReduce = false;
+  const T* __restrict__ b;
+  T* __restrict__ r;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    r[i] = b[i] - s;
+    return 0.0;
+  }
+  __device__ void finalize(double) const {}
+};
+
+// is_converged(), conjugate_gradient.hpp:261-276
+__device__ inline void cg_check(CgState* st, double rr) {
+  double rn = sqrt(rr);
+  st->resnorm = rn;
+  if (st->collect) st->history[st->iters] = rn;
+  st->iters += 1;
+  if (rn / st->bnorm < st->tol) {
+    st->converged = 1;
+    st->done = 1;
+  }
+}
+
+template <typename T>
+void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
+              const spk_solver_config* cfg, const T* b, T* x,
+              spk_summary* out, double* history) {
+  SPK_REQUIRE(cfg->restart > 0, SPK_INVALID_ARGUMENT,
+              "restart_iterations must be > 0 (the reference divides by it)");
+  const int64_t n = A->nrows;
+  Workspace& ws = workspace();
+  const size_t vb = (size_t)(n > 0 ? n : 1) * sizeof(T);
+  T* r = (T*)ws.get(kWsCg + 0, vb);
+  T* z = (T*)ws.get(kWsCg + 1, vb);
+  T* p = (T*)ws.get(kWsCg + 2, vb);
+  T* q = (T*)ws.get(kWsCg + 3, vb);
+  const bool right_id = is_identity(MR);
+  T* hatp = right_id ? p : (T*)ws.get(kWsCg + 4, vb);
+  double* dhist = cfg->collect_residuals
+                      ? (double*)ws.get(kWsCg + 5, sizeof(double) *
+                                        (size_t)std::max<int64_t>(cfg->max_iterations, 1))
+                      : nullptr;
+  DevState<CgState> st(kWsCg + 6);
+  CgState h{};
+  h.tol = cfg->tolerance;
+  h.collect = cfg->collect_residuals ? 1 : 0;
+  h.history = dhist;
+  st.upload(h);
+  CgState* S = st.d;
+  hipStream_t s = stream();
+
+  // bnorm = sqrt(<b,b>) (:194); zero rhs -> x = 0, converged (:195-199)
+  launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+    acc[0] += (double)b[i] * (double)b[i];
+  }, [=] __device__(const double* t) { S->bnorm = sqrt(t[0]); });
+  h = st.download();
+  if (h.bnorm == 0.0) {
+    launch_map(n, [=] __device__(int64_t i) { x[i] = T(0); });
+    out->converged = 1;
+    out->residual_norm = 0.0;
+    out->computed_iterations = 0;
+    out->status = SPK_OK;
+    out->history_len = 0;
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    return;
+  }
+
+  const bool jac = is_jacobi(ML);
+  const T* invd = jac ? ML->inv_diag<T>() : nullptr;
+  const int* done = &S->done;
+
+  auto issue = [&](int64_t k) {
+    if (k % cfg->restart == 0) {  // reorthogonalize(), :216-230
+      if (jac) {
+        launch_spmv(*A, (const T*)x, EpiCgRestartJacobi<T>{b, invd, r, p, S, done}, s);
+      } else {
+        launch_spmv(*A, (const T*)x, EpiCgResidual<T>{b, r, done}, s);
+        apply_precond(ML, n, (const T*)r, z);
+        launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+          T zi = z[i];
+          p[i] = zi;
+          acc[0] += (double)r[i] * (double)zi;
+        }, [=] __device__(const double* t) { S->rz = t[0]; }, done);
+      }
+    }
+    // compute_step_size(), :232-239
+    if (!right_id) apply_precond(MR, n, (const T*)p, hatp);
+    launch_spmv(*A, (const T*)hatp, EpiCgDot<T>{p, q, S, done}, s);
+    if (jac) {
+      // compute_new_solution/residual + is_converged + the z, rz_new of
+      // update_direction (:241-292), fused
+      launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {
+        const T a = (T)S->alpha;
+        x[i] = a * hatp[i] + x[i];
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)ri * (double)(invd[i] * ri);
+      }, [=] __device__(const double* t) {
+        cg_check(S, t[0]);
+        if (!S->done) {
+          S->beta = t[1] / S->rz;
+          S->rz = t[1];
+        }
+      }, done);
+      launch_map(n, [=] __device__(int64_t i) {
+        p[i] = invd[i] * r[i] + (T)S->beta * p[i];
+      }, done);
+    } else {
+      launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+        const T a = (T)S->alpha;
+        x[i] = a * hatp[i] + x[i];
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+      }, [=] __device__(const double* t) { cg_check(S, t[0]); }, done);
+      apply_precond(ML, n, (const T*)r, z);
+      launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+        acc[0] += (double)r[i] * (double)z[i];
+      }, [=] __device__(const double* t) {
+        S->beta = t[0] / S->rz;
+        S->rz = t[0];
+      }, done);
+      launch_map(n, [=] __device__(int64_t i) {
+        p[i] = z[i] + (T)S->beta * p[i];
+      }, done);
+    }
+  };
+  auto is_done = [&]() { return st.download().done != 0; };
+  chunked_loop(cfg->max_iterations, issue, is_done);
+
+  h = st.download();
+  out->residual_norm = h.resnorm;
+  out->computed_iterations = h.iters;
+  out->converged = h.converged;
+  out->status = SPK_OK;
+  out->history_len = h.collect ? h.iters : 0;
+  if (history && h.collect && h.iters > 0) {
+    SPK_HIP_TRY(hipMemcpyAsync(history, dhist, h.iters * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" spk_status spk_cg_solve(spk_csr A, spk_precond M_left,
+                                   spk_precond M_right,
+                                   const spk_solver_config* cfg,
+                                   const void* d_b, void* d_x,
+                                   spk_summary* summary, double* history) {
+  return guarded([&] {
+    check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
+    SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
+    *summary = spk_summary{};
+    if (cfg->precision == SPK_F32)
+      cg_solve<float>(A, M_left, M_right, cfg, (const float*)d_b, (float*)d_x,
+                      summary, history);
+    else
+      cg_solve<double>(A, M_left, M_right, cfg, (const double*)d_b,
+                       (double*)d_x, summary, history);
+  });
+}

@@ -1,0 +1,293 @@
+// spk_csr.hip -- compressed_row store: upload, plan, SpMV entry point,
+// diagonal extraction and download.
+#include "spk_csr.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace spk {
+
+std::vector<int32_t> build_spmv_plan(const int32_t* rp, int64_t nrows,
+                                     int* nlong) {
+  std::vector<int32_t> blocks;
+  blocks.reserve((size_t)(nrows / 64 + 2));
+  blocks.push_back(0);
+  int64_t r = 0;
+  int longs = 0;
+  while (r < nrows) {
+    int64_t len = (int64_t)rp[r + 1] - rp[r];
+    if (len > kNnzBlock) {
+      ++r;
+      ++longs;
+      blocks.push_back((int32_t)r);
+      continue;
+    }
+    int64_t start = r, nnz = 0;
+    while (r < nrows && r - start < kSpmvBlock) {
+      int64_t l = (int64_t)rp[r + 1] - rp[r];
+      if (nnz + l > kNnzBlock) break;
+      nnz += l;
+      ++r;
+    }
+    blocks.push_back((int32_t)r);
+  }
+  *nlong = longs;
+  return blocks;
+}
+
+namespace {
+
+__global__ void check_cols_kernel(const int32_t* ci, int64_t nnz, int64_t ncols,
+                                  int* bad) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    int32_t c = ci[k];
+    if (c < 0 || c >= ncols) atomicOr(bad, 1);
+  }
+}
+
+__global__ void f64_to_f32_kernel(const double* in, float* out, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = (float)in[k];
+}
+
+// extract_diagonal: one thread per row, binary search (unary.hpp:29-37)
+__global__ void diag_kernel(const int32_t* rp, const int32_t* ci,
+                            const double* v, int64_t len, double* d) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  int lo = rp[i], hi = rp[i + 1], e = hi;
+  while (lo < hi) {
+    int mid = lo + (hi - lo) / 2;
+    if (ci[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  d[i] = (lo < e && ci[lo] == i) ? v[lo] : 0.0;
+}
+
+int grid_for(int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 65536);
+}
+
+}  // namespace
+
+void finish_store(spk_csr_s& A, bool validate_cols) {
+  hipStream_t s = stream();
+  A.h_rp.resize((size_t)A.nrows + 1);
+  SPK_HIP_TRY(hipMemcpyAsync(A.h_rp.data(), A.rp.p,
+                             (A.nrows + 1) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s));
+  SPK_HIP_TRY(hipStreamSynchronize(s));
+  SPK_REQUIRE(A.h_rp[0] == 0 && A.h_rp[A.nrows] == A.nnz, SPK_INVALID_ARGUMENT,
+              "row_ptr must start at 0 and end at nnz");
+  for (int64_t r = 0; r < A.nrows; ++r)
+    SPK_REQUIRE(A.h_rp[r] <= A.h_rp[r + 1], SPK_INVALID_ARGUMENT,
+                "row_ptr must be non-decreasing");
+  if (validate_cols && A.nnz > 0) {
+    DevBuf<int> bad(1);
+    SPK_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(check_cols_kernel, dim3(grid_for(A.nnz)), dim3(256), 0, s,
+                       A.ci.p, A.nnz, A.ncols, bad.p);
+    int hbad = 0;
+    SPK_HIP_TRY(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    SPK_REQUIRE(!hbad, SPK_INVALID_ARGUMENT, "column index out of range");
+  }
+  upload_plan(A, build_spmv_plan(A.h_rp.data(), A.nrows, &A.nlong));
+}
+
+void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks) {
+  std::vector<int2> plan(blocks.size());
+  for (size_t b = 0; b < blocks.size(); ++b)
+    plan[b] = make_int2(blocks[b], A.h_rp[blocks[b]]);
+  A.nblocks = (int)blocks.size() - 1;
+  A.plan.alloc(plan.size());
+  SPK_HIP_TRY(hipMemcpyAsync(A.plan.p, plan.data(), plan.size() * sizeof(int2),
+                             hipMemcpyHostToDevice, stream()));
+  SPK_HIP_TRY(hipStreamSynchronize(stream()));
+}
+
+}  // namespace spk
+
+const float* spk_csr_s::vals32() {
+  if (!has32) {
+    v32.alloc((size_t)nnz);
+    hipStream_t s = spk::stream();
+    if (nnz > 0)
+      hipLaunchKernelGGL(spk::f64_to_f32_kernel, dim3(spk::grid_for(nnz)),
+                         dim3(256), 0, s, v64.p, v32.p, nnz);
+    SPK_HIP_TRY(hipGetLastError());
+    has32 = true;
+  }
+  return v32.p;
+}
+
+using namespace spk;
+
+extern "C" {
+
+spk_status spk_csr_create(int64_t nrows, int64_t ncols, int64_t nnz,
+                          const int64_t* row_ptr, const int64_t* col_ind,
+                          const double* vals, spk_csr* out) {
+  return guarded([&] {
+    SPK_REQUIRE(out && row_ptr, SPK_INVALID_ARGUMENT, "null argument");
+    SPK_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, SPK_INVALID_ARGUMENT,
+                "negative size");
+    SPK_REQUIRE(nrows < INT_MAX && nnz < INT_MAX, SPK_INVALID_ARGUMENT,
+                "nrows and nnz must fit in int32 for the device store");
+    SPK_REQUIRE(nnz == 0 || (col_ind && vals), SPK_INVALID_ARGUMENT,
+                "null col_ind/vals");
+    std::vector<int32_t> rp32((size_t)nrows + 1), ci32((size_t)nnz);
+    for (int64_t r = 0; r <= nrows; ++r) rp32[r] = (int32_t)row_ptr[r];
+    SPK_REQUIRE(row_ptr[nrows] == nnz, SPK_INVALID_ARGUMENT,
+                "row_ptr[nrows] != nnz");
+    for (int64_t k = 0; k < nnz; ++k) {
+      SPK_REQUIRE(col_ind[k] >= 0 && col_ind[k] < ncols, SPK_INVALID_ARGUMENT,
+                  "column index out of range");
+      ci32[k] = (int32_t)col_ind[k];
+    }
+    auto* A = new spk_csr_s();
+    try {
+      A->nrows = nrows;
+      A->ncols = ncols;
+      A->nnz = nnz;
+      A->rp.alloc((size_t)nrows + 1);
+      A->ci.alloc((size_t)nnz);
+      A->v64.alloc((size_t)nnz);
+      hipStream_t s = stream();
+      SPK_HIP_TRY(hipMemcpyAsync(A->rp.p, rp32.data(), rp32.size() * 4,
+                                 hipMemcpyHostToDevice, s));
+      if (nnz) {
+        SPK_HIP_TRY(hipMemcpyAsync(A->ci.p, ci32.data(), ci32.size() * 4,
+                                   hipMemcpyHostToDevice, s));
+        SPK_HIP_TRY(hipMemcpyAsync(A->v64.p, vals, (size_t)nnz * 8,
+                                   hipMemcpyHostToDevice, s));
+      }
+      finish_store(*A, false);
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = A;
+  });
+}
+
+spk_status spk_csr_create_device(int64_t nrows, int64_t ncols, int64_t nnz,
+                                 const int32_t* d_row_ptr,
+                                 const int32_t* d_col_ind,
+                                 const double* d_vals, spk_csr* out) {
+  return guarded([&] {
+    SPK_REQUIRE(out && d_row_ptr, SPK_INVALID_ARGUMENT, "null argument");
+    SPK_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0 && nrows < INT_MAX &&
+                    nnz < INT_MAX, SPK_INVALID_ARGUMENT, "bad size");
+    auto* A = new spk_csr_s();
+    try {
+      A->nrows = nrows;
+      A->ncols = ncols;
+      A->nnz = nnz;
+      A->rp.alloc((size_t)nrows + 1);
+      A->ci.alloc((size_t)nnz);
+      A->v64.alloc((size_t)nnz);
+      hipStream_t s = stream();
+      SPK_HIP_TRY(hipMemcpyAsync(A->rp.p, d_row_ptr, (nrows + 1) * 4,
+                                 hipMemcpyDeviceToDevice, s));
+      if (nnz) {
+        SPK_HIP_TRY(hipMemcpyAsync(A->ci.p, d_col_ind, nnz * 4,
+                                   hipMemcpyDeviceToDevice, s));
+        SPK_HIP_TRY(hipMemcpyAsync(A->v64.p, d_vals, nnz * 8,
+                                   hipMemcpyDeviceToDevice, s));
+      }
+      finish_store(*A, true);
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = A;
+  });
+}
+
+spk_status spk_csr_destroy(spk_csr A) {
+  return guarded([&] {
+    if (!A) return;
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    delete A;
+  });
+}
+
+spk_status spk_csr_shape(spk_csr A, int64_t* nrows, int64_t* ncols,
+                         int64_t* nnz) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    if (nrows) *nrows = A->nrows;
+    if (ncols) *ncols = A->ncols;
+    if (nnz) *nnz = A->nnz;
+  });
+}
+
+spk_status spk_csr_download(spk_csr A, int64_t* row_ptr, int64_t* col_ind,
+                            double* vals) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    hipStream_t s = stream();
+    if (row_ptr)
+      for (int64_t r = 0; r <= A->nrows; ++r) row_ptr[r] = A->h_rp[r];
+    if (col_ind && A->nnz) {
+      std::vector<int32_t> ci((size_t)A->nnz);
+      SPK_HIP_TRY(hipMemcpyAsync(ci.data(), A->ci.p, A->nnz * 4,
+                                 hipMemcpyDeviceToHost, s));
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      for (int64_t k = 0; k < A->nnz; ++k) col_ind[k] = ci[k];
+    }
+    if (vals && A->nnz) {
+      SPK_HIP_TRY(hipMemcpyAsync(vals, A->v64.p, A->nnz * 8,
+                                 hipMemcpyDeviceToHost, s));
+    }
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+  });
+}
+
+spk_status spk_csr_device_arrays(spk_csr A, int dtype, const int32_t** rp,
+                                 const int32_t** ci, const void** vals) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    if (rp) *rp = A->rp.p;
+    if (ci) *ci = A->ci.p;
+    if (vals) *vals = dtype == SPK_F32 ? (const void*)A->vals32()
+                                       : (const void*)A->v64.p;
+  });
+}
+
+spk_status spk_csr_plan_info(spk_csr A, int64_t* nblocks,
+                             int64_t* nlong_blocks) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    if (nblocks) *nblocks = A->nblocks;
+    if (nlong_blocks) *nlong_blocks = A->nlong;
+  });
+}
+
+spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y) {
+  return guarded([&] {
+    SPK_REQUIRE(A && d_x && d_y, SPK_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = stream();
+    if (dtype == SPK_F32)
+      launch_spmv(*A, (const float*)d_x, EpiStore<float>{(float*)d_y}, s);
+    else
+      launch_spmv(*A, (const double*)d_x, EpiStore<double>{(double*)d_y}, s);
+  });
+}
+
+spk_status spk_csr_extract_diagonal(spk_csr A, double* d_diag) {
+  return guarded([&] {
+    SPK_REQUIRE(A && d_diag, SPK_INVALID_ARGUMENT, "null argument");
+    int64_t len = std::min(A->nrows, A->ncols);
+    if (len == 0) return;
+    hipLaunchKernelGGL(diag_kernel, dim3((unsigned)((len + 255) / 256)),
+                       dim3(256), 0, stream(), A->rp.p, A->ci.p, A->v64.p, len,
+                       d_diag);
+    SPK_HIP_TRY(hipGetLastError());
+  });
+}
+
+}  // extern "C"

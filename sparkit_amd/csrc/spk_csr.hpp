@@ -1,0 +1,201 @@
+// spk_csr.hpp -- the device-resident compressed_row store, its row-binned
+// SpMV plan and the SpMV kernel template shared by the solvers.
+//
+// Layout in HBM (one store):
+//   rp[nrows+1]  int32   row offsets      (Compressed_row_sparsity::row_ptr)
+//   ci[nnz]      int32   column indices   (Compressed_row_sparsity::col_ind)
+//   v64[nnz]     double  values           (Compressed_row_matrix::values)
+//   v32[nnz]     float   FP32 copy, made on first FP32 use
+//   plan[nb+1]   int2    {first row, first nonzero} of each SpMV row block
+//
+// Plan: rows are cut greedily into blocks of at most kSpmvBlock rows and at
+// most kNnzBlock nonzeros; a row longer than kNnzBlock gets a block of its
+// own (a "long" block).  One workgroup of kSpmvBlock threads runs per block:
+//  - short/medium rows: the block's nonzeros are streamed with coalesced
+//    loads of ci/v, multiplied by the gathered x and staged in LDS; then each
+//    row is summed from LDS by L lanes (L = largest power of two with
+//    rows * L <= kSpmvBlock, at most 64).  With L = 1 a lane adds its row's
+//    products in storage order, which is sparse_blas.hpp:26-31's order, so
+//    FP64 results are bit-identical to the reference for such rows.
+//  - long rows: all threads stride over the row, then a block reduction.
+#pragma once
+
+#include "spk_internal.hpp"
+#include "spk_vec.hpp"
+
+namespace spk {
+
+constexpr int kSpmvBlock = 256;
+constexpr int kNnzBlock = 2048;
+
+template <typename T>
+struct CsrView {
+  const int32_t* __restrict__ rp;
+  const int32_t* __restrict__ ci;
+  const T* __restrict__ v;
+  const int2* __restrict__ plan;  // {first row, first nonzero} per block
+  int nblocks;
+};
+
+}  // namespace spk
+
+struct spk_csr_s {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  spk::DevBuf<int32_t> rp, ci;
+  spk::DevBuf<int2> plan;
+  spk::DevBuf<double> v64;
+  spk::DevBuf<float> v32;
+  bool has32 = false;
+  int nblocks = 0, nlong = 0;
+  std::vector<int32_t> h_rp;  // host copy of the row offsets
+
+  const float* vals32();
+  template <typename T>
+  spk::CsrView<T> view();
+};
+
+template <>
+inline spk::CsrView<double> spk_csr_s::view<double>() {
+  return {rp.p, ci.p, v64.p, plan.p, nblocks};
+}
+template <>
+inline spk::CsrView<float> spk_csr_s::view<float>() {
+  return {rp.p, ci.p, vals32(), plan.p, nblocks};
+}
+
+namespace spk {
+
+// Build the row-block plan from host row offsets: row-block boundaries.
+std::vector<int32_t> build_spmv_plan(const int32_t* rp, int64_t nrows,
+                                     int* nlong);
+// Upload a plan ({first row, first nonzero} per boundary) into A.plan.
+void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks);
+// Finish a store whose rp/ci/v64 are filled on the device: copies rp back,
+// validates it and builds the plan.
+void finish_store(spk_csr_s& A, bool validate_cols);
+
+// ------------------------------------------------------------- epilogues
+//
+// An epilogue receives each row's sum: row(i, s) stores what it needs and
+// returns the row's contribution to an optional grid-wide sum (kReduce);
+// finalize(total) runs once, in the finisher kernel.  `done` (may be null)
+// is a device flag that turns the launch into a no-op.
+
+// y = A x
+template <typename T>
+struct EpiStore {
+  static constexpr bool kReduce = false;
+  T* __restrict__ y;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const { y[i] = s; return 0.0; }
+  __device__ void finalize(double) const {}
+};
+
+// y = b - A x
+template <typename T>
+struct EpiResidual {
+  static constexpr bool kReduce = false;
+  const T* __restrict__ b;
+  T* __restrict__ y;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const { y[i] = b[i] - s; return 0.0; }
+  __device__ void finalize(double) const {}
+};
+
+template <typename Epi>
+struct EpiFinalize {  // adapts Epi::finalize to the finisher's fin(totals)
+  Epi epi;
+  __device__ void operator()(const double* t) const { epi.finalize(t[0]); }
+};
+
+// ------------------------------------------------------------- kernel
+
+template <typename T, typename Epi>
+__global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
+    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials,
+    int nxcd) {
+  __shared__ T prod[kNnzBlock];
+  __shared__ double red[kSpmvBlock / 64];
+  if (epi.done && *epi.done) return;
+  const int tid = threadIdx.x;
+  const int b = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+  const int2 p0 = A.plan[b], p1 = A.plan[b + 1];
+  const int r0 = p0.x, base = p0.y, nrows = p1.x - p0.x, cnt = p1.y - p0.y;
+  double acc = 0.0;
+  if (cnt > kNnzBlock) {
+    // long row: the block holds exactly one row
+    T s = T(0);
+    for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
+      s += A.v[k] * x[A.ci[k]];
+    s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
+    if (tid == 0) acc = epi.row(r0, s);
+  } else {
+    // lanes per row: enough rows to occupy the block, at least ~8 products
+    // per lane; short rows (stencils) get one lane each
+    int L = 1;
+    while (L < 64 && 2 * L * nrows <= kSpmvBlock && 16 * L * nrows <= cnt) L *= 2;
+    constexpr int U = kNnzBlock / kSpmvBlock;
+    // issue every index/value load of the block, and the row bounds, before
+    // the gathers: three dependent memory trips per block in total
+    int c[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = tid + u * kSpmvBlock;
+      const bool ok = k < cnt;
+      c[u] = ok ? A.ci[base + k] : 0;
+      a[u] = ok ? A.v[base + k] : T(0);
+    }
+    const int g = tid / L, lane = tid % L;
+    int s0 = 0, s1 = 0;
+    if (g < nrows) {
+      s0 = A.rp[r0 + g] - base;
+      s1 = A.rp[r0 + g + 1] - base;
+    }
+    T xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      xv[u] = (tid + u * kSpmvBlock < cnt) ? x[c[u]] : T(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = tid + u * kSpmvBlock;
+      if (k < cnt) prod[k] = a[u] * xv[u];
+    }
+    __syncthreads();
+    if (L == 1) {
+      // storage-order sum (sparse_blas.hpp:26-31)
+      if (g < nrows) {
+        T s = T(0);
+        for (int j = s0; j < s1; ++j) s += prod[j];
+        acc = epi.row(r0 + g, s);
+      }
+    } else {
+      T s = T(0);
+      if (g < nrows)
+        for (int j = s0 + lane; j < s1; j += L) s += prod[j];
+      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (g < nrows && lane == 0) acc = epi.row(r0 + g, s);
+    }
+  }
+  if constexpr (Epi::kReduce) {
+    double t = block_reduce_sum<double, kSpmvBlock>(acc, red);
+    if (tid == 0) partials[b] = t;
+  }
+}
+
+template <typename T, typename Epi>
+void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
+  if (A.nrows == 0) return;
+  double* parts = Epi::kReduce ? reduce_partials(A.nblocks) : nullptr;
+  hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
+                     0, s, A.view<T>(), x, epi, parts, ctx().num_xcds);
+  SPK_HIP_TRY(hipGetLastError());
+  if constexpr (Epi::kReduce) {
+    hipLaunchKernelGGL((finish_kernel<1, EpiFinalize<Epi>>), dim3(1),
+                       dim3(kFinishBlock), 0, s, (const double*)parts,
+                       A.nblocks, EpiFinalize<Epi>{epi}, epi.done);
+    SPK_HIP_TRY(hipGetLastError());
+  }
+}
+
+}  // namespace spk

@@ -1,0 +1,307 @@
+// spk_gmres.hip -- device GMRES(m) (gmres.hpp:99-375) and FGMRES(m)
+// (fgmres.hpp:96-428).
+//
+// One restart cycle is queued by the host without reading anything back:
+//   r0 = b - A x (SpMV epilogue); GMRES: V0 = M_L r0; beta = |V0|
+//   Arnoldi step j:  Z_j = M_R V_j (GMRES) / M V_j (FGMRES)
+//                    V_{j+1} = M_L A Z_j (GMRES) / A Z_j (FGMRES)
+//                    modified Gram-Schmidt: j+1 fused kernels, kernel i
+//                    applies the update of V_{i-1} and reduces <V_i, V_{j+1}>
+//                    (so each MGS sweep streams V_{j+1} once per basis
+//                    vector, in the reference's order)
+//                    last kernel: update with V_j and |V_{j+1}|^2; its last
+//                    block runs the Givens rotations and the convergence test
+//                    on the device (FP64), then V_{j+1} is normalised
+//   form_solution:   back-substitution (one thread) and x += Z y
+// The host reads the state once per cycle.  Vectors may be FP32
+// (precision = SPK_F32); H, the rotations, g, y, every dot product and the
+// Gram-Schmidt updates are computed in FP64 and the basis is stored in T.
+#include "spk_solver_common.hpp"
+
+namespace spk {
+namespace {
+
+struct GmState {
+  double bnorm, tol, resnorm, hnorm;
+  int64_t iters;
+  int32_t stop;       // this cycle is finished (converged / breakdown)
+  int32_t done;       // the solve is finished
+  int32_t converged;
+  int32_t collect;
+  int32_t j_final;
+  int32_t flexible;
+  double* history;
+  double* H;  // (m+1) x m, column-major, leading dimension m+1
+  double* cs;
+  double* sn;
+  double* g;
+  double* y;
+  int64_t m;
+};
+
+// apply_givens_rotations + residual record + convergence check
+// (gmres.hpp:283-326, 193-205; fgmres.hpp:286-356), thread 0 of the last
+// block of the final Gram-Schmidt kernel of step j.
+__device__ void arnoldi_finish(GmState* st, int j, double h) {
+  const int64_t ld = st->m + 1;
+  double* col = st->H + (int64_t)j * ld;
+  double* cs = st->cs;
+  double* sn = st->sn;
+  double* g = st->g;
+  col[j + 1] = h;
+  st->hnorm = h;
+  for (int i = 0; i < j; ++i) {
+    double temp = cs[i] * col[i] + sn[i] * col[i + 1];
+    col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+    col[i] = temp;
+  }
+  double a = col[j], b = col[j + 1];
+  if (b == 0.0) {
+    cs[j] = 1.0;
+    sn[j] = 0.0;
+  } else if (a == 0.0) {
+    cs[j] = 0.0;
+    sn[j] = 1.0;
+  } else {
+    double r = sqrt(a * a + b * b);
+    cs[j] = a / r;
+    sn[j] = b / r;
+  }
+  col[j] = cs[j] * col[j] + sn[j] * col[j + 1];
+  col[j + 1] = 0.0;
+  double temp = cs[j] * g[j] + sn[j] * g[j + 1];
+  g[j + 1] = -sn[j] * g[j] + cs[j] * g[j + 1];
+  g[j] = temp;
+  st->iters += 1;
+  st->j_final = j;
+  double absres = fabs(g[j + 1]);
+  st->resnorm = absres;
+  if (st->collect) st->history[st->iters - 1] = absres;
+  if (absres / st->bnorm < st->tol) {
+    st->converged = 1;
+    st->stop = 1;
+  }
+}
+
+// form_solution's back-substitution R y = g (gmres.hpp:333-342)
+__global__ void back_substitute_kernel(GmState* st) {
+  if (st->done || threadIdx.x != 0) return;
+  const int64_t ld = st->m + 1;
+  const int nj = st->j_final + 1;
+  const double* H = st->H;
+  double* y = st->y;
+  for (int ii = nj; ii > 0; --ii) {
+    int i = ii - 1;
+    double yi = st->g[i];
+    for (int k = i + 1; k < nj; ++k) yi -= H[i + (int64_t)k * ld] * y[k];
+    y[i] = yi / H[i + (int64_t)i * ld];
+  }
+}
+
+template <typename T>
+void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
+                 spk_precond_s* MR, const spk_solver_config* cfg, const T* b,
+                 T* x, spk_summary* out, double* history) {
+  SPK_REQUIRE(cfg->restart > 0, SPK_INVALID_ARGUMENT,
+              "restart_dimension must be > 0");
+  const int64_t n = A->nrows, m = cfg->restart, ld = m + 1;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  Workspace& ws = workspace();
+  T* V = (T*)ws.get(kWsGmres + 0, (size_t)(m + 1) * nn * sizeof(T));
+  // Z is needed whenever the right slot is not the identity
+  const bool right_id = flexible ? is_identity(ML) : is_identity(MR);
+  T* Z = right_id ? V : (T*)ws.get(kWsGmres + 1, (size_t)m * nn * sizeof(T));
+  T* r0 = (T*)ws.get(kWsGmres + 2, nn * sizeof(T));
+  double* small = (double*)ws.get(
+      kWsGmres + 3, sizeof(double) * (size_t)((m + 1) * m + 3 * m + 2 * (m + 1)));
+  double* dhist = cfg->collect_residuals
+                      ? (double*)ws.get(kWsGmres + 4, sizeof(double) *
+                                        (size_t)std::max<int64_t>(cfg->max_iterations, 1))
+                      : nullptr;
+  DevState<GmState> st(kWsGmres + 5);
+  GmState h{};
+  h.tol = cfg->tolerance;
+  h.collect = cfg->collect_residuals ? 1 : 0;
+  h.flexible = flexible;
+  h.history = dhist;
+  h.H = small;
+  h.cs = small + (m + 1) * m;
+  h.sn = h.cs + m;
+  h.g = h.sn + m;
+  h.y = h.g + (m + 1);
+  h.m = m;
+  st.upload(h);
+  GmState* S = st.d;
+  hipStream_t s = stream();
+  const int* stop = &S->stop;
+  const int* done = &S->done;
+  spk_precond_s* left = flexible ? nullptr : ML;
+  spk_precond_s* right = flexible ? ML : MR;
+
+  launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+    acc[0] += (double)b[i] * (double)b[i];
+  }, [=] __device__(const double* t) { S->bnorm = sqrt(t[0]); });
+  h = st.download();
+  if (h.bnorm == 0.0) {  // gmres.hpp:165-170
+    launch_map(n, [=] __device__(int64_t i) { x[i] = T(0); });
+    *out = spk_summary{0.0, 0, 1, SPK_OK, 0};
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    return;
+  }
+
+  int64_t iters = 0;
+  while (iters < cfg->max_iterations && !h.converged && !h.done) {
+    // compute_initial_residual (gmres.hpp:211-241 / fgmres.hpp:206-224)
+    if (is_identity(left)) {
+      launch_spmv(*A, (const T*)x, EpiResidual<T>{b, V}, s);
+    } else {
+      launch_spmv(*A, (const T*)x, EpiResidual<T>{b, r0}, s);
+      left->apply((const T*)r0, V);
+    }
+    launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+      acc[0] += (double)V[i] * (double)V[i];
+    }, [=] __device__(const double* t) {
+      double beta = sqrt(t[0]);
+      for (int64_t k = 0; k <= S->m; ++k) S->g[k] = 0.0;
+      for (int64_t k = 0; k < (S->m + 1) * S->m; ++k) S->H[k] = 0.0;
+      for (int64_t k = 0; k < S->m; ++k) S->cs[k] = S->sn[k] = 0.0;
+      S->g[0] = beta;
+      S->hnorm = beta;
+      S->j_final = 0;
+      S->stop = 0;
+      if (beta == 0.0) {  // x is exact; the reference would divide by zero
+        S->resnorm = 0.0;
+        S->converged = 1;
+        S->stop = 1;
+        S->done = 1;
+      }
+    });
+    if (flexible) {
+      launch_map(n, [=] __device__(int64_t i) {
+        V[i] = (T)((double)V[i] * (1.0 / S->hnorm));
+      }, stop);
+    } else {
+      launch_map(n, [=] __device__(int64_t i) {
+        V[i] = (T)((double)V[i] / S->hnorm);
+      }, stop);
+    }
+    const int64_t m_eff = std::min(m, cfg->max_iterations - iters);
+    for (int64_t j = 0; j < m_eff; ++j) {
+      T* vj = V + (size_t)j * nn;
+      T* vj1 = V + (size_t)(j + 1) * nn;
+      T* zj = right_id ? vj : Z + (size_t)j * nn;
+      if (!right_id) right->apply((const T*)vj, zj);
+      if (is_identity(left)) {
+        launch_spmv(*A, (const T*)zj, EpiStore<T>{vj1, stop}, s);
+      } else {
+        launch_spmv(*A, (const T*)zj, EpiStore<T>{r0, stop}, s);
+        left->apply((const T*)r0, vj1);
+      }
+      // modified Gram-Schmidt (gmres.hpp:258-267)
+      for (int64_t i = 0; i <= j; ++i) {
+        const T* vi = V + (size_t)i * nn;
+        const T* vim1 = i > 0 ? V + (size_t)(i - 1) * nn : nullptr;
+        double* hcol = small + j * ld;
+        const int64_t ii = i;
+        launch_reduce<1>(n, [=] __device__(int64_t e, double* acc) {
+          T w = vj1[e];
+          if (ii > 0) {
+            w = (T)((double)w - hcol[ii - 1] * (double)vim1[e]);
+            vj1[e] = w;
+          }
+          acc[0] += (double)vi[e] * (double)w;
+        }, [=] __device__(const double* t) { hcol[ii] = t[0]; }, stop);
+      }
+      {
+        const double* hcol = small + j * ld;
+        const int jj = (int)j;
+        launch_reduce<1>(n, [=] __device__(int64_t e, double* acc) {
+          T w = (T)((double)vj1[e] - hcol[jj] * (double)vj[e]);
+          vj1[e] = w;
+          acc[0] += (double)w * (double)w;
+        }, [=] __device__(const double* t) {
+          arnoldi_finish(S, jj, sqrt(t[0]));
+        }, stop);
+      }
+      // normalise V_{j+1} if h > 0 (gmres.hpp:276-280 divides,
+      // fgmres.hpp:283 multiplies by the reciprocal)
+      if (flexible) {
+        launch_map(n, [=] __device__(int64_t e) {
+          double hn = S->hnorm;
+          if (hn > 0.0) vj1[e] = (T)((double)vj1[e] * (1.0 / hn));
+        }, stop);
+      } else {
+        launch_map(n, [=] __device__(int64_t e) {
+          double hn = S->hnorm;
+          if (hn > 0.0) vj1[e] = (T)((double)vj1[e] / hn);
+        }, stop);
+      }
+    }
+    // form_solution (gmres.hpp:328-354)
+    hipLaunchKernelGGL(back_substitute_kernel, dim3(1), dim3(64), 0, s, S);
+    const T* Zc = Z;
+    launch_map(n, [=] __device__(int64_t e) {
+      const int nj = S->j_final + 1;
+      double sum = 0.0;
+      for (int k = 0; k < nj; ++k) sum += (double)Zc[(size_t)k * nn + e] * S->y[k];
+      x[e] = (T)((double)x[e] + sum);
+    }, done);
+    // close the cycle: a converged cycle finishes the solve
+    launch_map(1, [=] __device__(int64_t) {
+      if (S->converged) S->done = 1;
+    }, done);
+    h = st.download();
+    iters = h.iters;
+  }
+  out->residual_norm = h.resnorm;
+  out->computed_iterations = h.iters;
+  out->converged = h.converged;
+  out->status = SPK_OK;
+  out->history_len = h.collect ? h.iters : 0;
+  if (history && h.collect && h.iters > 0) {
+    SPK_HIP_TRY(hipMemcpyAsync(history, dhist, h.iters * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" {
+
+spk_status spk_gmres_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
+                           const spk_solver_config* cfg, const void* d_b,
+                           void* d_x, spk_summary* summary, double* history) {
+  return guarded([&] {
+    check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
+    SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
+    *summary = spk_summary{};
+    if (cfg->precision == SPK_F32)
+      gmres_solve<float>(false, A, M_left, M_right, cfg, (const float*)d_b,
+                         (float*)d_x, summary, history);
+    else
+      gmres_solve<double>(false, A, M_left, M_right, cfg, (const double*)d_b,
+                          (double*)d_x, summary, history);
+  });
+}
+
+spk_status spk_fgmres_solve(spk_csr A, spk_precond M,
+                            const spk_solver_config* cfg, const void* d_b,
+                            void* d_x, spk_summary* summary, double* history) {
+  return guarded([&] {
+    check_solver_args(A, M, nullptr, cfg, d_b, d_x);
+    SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
+    *summary = spk_summary{};
+    if (cfg->precision == SPK_F32)
+      gmres_solve<float>(true, A, M, nullptr, cfg, (const float*)d_b,
+                         (float*)d_x, summary, history);
+    else
+      gmres_solve<double>(true, A, M, nullptr, cfg, (const double*)d_b,
+                          (double*)d_x, summary, history);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// spk_precond.hpp -- device preconditioners: identity, Jacobi, ILU(0)/MILU(0)
+// and "apply an explicit sparse approximate inverse" (one SpMV).
+//
+// ILU(0) layout in HBM:
+//   L store: strictly-lower entries of A's pattern + unit diagonal LAST in
+//            each row; U store: diagonal FIRST + upper entries (the
+//            reference's layout, ilu.hpp:123-141), int32 indices.
+//   lvl_rows_l / lvl_ptr_l: rows of the forward sweep (and of the numeric
+//            factorization) grouped by dependency level;
+//   lvl_rows_u / lvl_ptr_u: the same for the backward sweep.
+// A level is one launch; rows in a level are independent.  The numeric
+// factorization runs the reference's IKJ loop (ilu.hpp:72-115) for every row
+// of a level in one thread, with contraction of multiply-adds disabled, so
+// the factors are bit-identical to the sequential reference.
+#pragma once
+
+#include "spk_csr.hpp"
+
+struct spk_precond_s {
+  enum Kind { kIdentity, kJacobi, kIlu, kSpmv } kind = kIdentity;
+  int64_t n = 0;
+  // Jacobi
+  spk::DevBuf<double> inv_d64;
+  spk::DevBuf<float> inv_d32;
+  bool has32 = false;
+  // ILU(0)
+  spk_csr_s* L = nullptr;
+  spk_csr_s* U = nullptr;
+  spk::DevBuf<int32_t> lvl_rows_l, lvl_rows_u;
+  std::vector<int32_t> lvl_ptr_l, lvl_ptr_u;
+  spk::DevBuf<double> tmp64;
+  spk::DevBuf<float> tmp32;
+  // SPAI
+  spk_csr_s* M = nullptr;  // not owned
+
+  ~spk_precond_s();
+  const float* inv_diag32();
+  template <typename T>
+  const T* inv_diag();
+  template <typename T>
+  void apply(const T* in, T* out);
+  template <typename T>
+  void forward(const T* b, T* x);
+  template <typename T>
+  void backward(const T* b, T* x);
+  template <typename T>
+  T* tmp();
+};
+
+template <>
+inline const double* spk_precond_s::inv_diag<double>() { return inv_d64.p; }
+template <>
+inline const float* spk_precond_s::inv_diag<float>() { return inv_diag32(); }

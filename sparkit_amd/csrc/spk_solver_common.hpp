@@ -1,0 +1,70 @@
+// spk_solver_common.hpp -- pieces shared by the device Krylov solvers:
+// workspace slot map, argument checks and the chunked issue loop.
+#pragma once
+
+#include "spk_precond.hpp"
+#include "spk_vec.hpp"
+
+namespace spk {
+
+// Workspace slots (each solver owns a range so vectors stay allocated
+// between calls and the timed region performs no hipMalloc).
+enum : int {
+  kWsCg = 0,         // 0..9
+  kWsGmres = 10,     // 10..29
+  kWsBicgstab = 30,  // 30..44
+  kWsIr = 45,        // 45..59
+  kWsMisc = 60,
+};
+
+inline bool is_identity(const spk_precond_s* M) {
+  return M == nullptr || M->kind == spk_precond_s::kIdentity;
+}
+inline bool is_jacobi(const spk_precond_s* M) {
+  return M != nullptr && M->kind == spk_precond_s::kJacobi;
+}
+
+template <typename T>
+inline void apply_precond(spk_precond_s* M, int64_t n, const T* in, T* out) {
+  if (is_identity(M)) {
+    if (in != out)
+      SPK_HIP_TRY(hipMemcpyAsync(out, in, (size_t)n * sizeof(T),
+                                 hipMemcpyDeviceToDevice, stream()));
+  } else {
+    M->apply(in, out);
+  }
+}
+
+inline void check_solver_args(spk_csr_s* A, spk_precond_s* ML,
+                              spk_precond_s* MR, const spk_solver_config* cfg,
+                              const void* b, void* x) {
+  SPK_REQUIRE(A && cfg && b && x, SPK_INVALID_ARGUMENT, "null argument");
+  SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
+              "solver requires a square operator");
+  SPK_REQUIRE(!ML || ML->n == A->nrows, SPK_INVALID_ARGUMENT,
+              "left preconditioner size mismatch");
+  SPK_REQUIRE(!MR || MR->n == A->nrows, SPK_INVALID_ARGUMENT,
+              "right preconditioner size mismatch");
+  SPK_REQUIRE(cfg->precision == SPK_F64 || cfg->precision == SPK_F32,
+              SPK_INVALID_ARGUMENT, "precision must be SPK_F64 or SPK_F32");
+  SPK_REQUIRE(cfg->max_iterations >= 0, SPK_INVALID_ARGUMENT,
+              "max_iterations must be >= 0");
+}
+
+// Host issue loop: iterations are queued in chunks without reading anything
+// back; after each chunk one small device-to-host copy tells whether the
+// device-side convergence flag is set.  Iterations queued after convergence
+// return immediately (their kernels test the flag).
+template <typename IssueOne, typename Done>
+void chunked_loop(int64_t max_it, IssueOne issue, Done is_done,
+                  int64_t first_chunk = 4, int64_t max_chunk = 64) {
+  int64_t k = 0, chunk = first_chunk;
+  while (k < max_it) {
+    int64_t kend = std::min(max_it, k + chunk);
+    for (; k < kend; ++k) issue(k);
+    if (is_done()) break;
+    chunk = std::min(chunk * 2, max_chunk);
+  }
+}
+
+}  // namespace spk

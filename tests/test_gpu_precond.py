@@ -1,0 +1,110 @@
+"""GPU parity of the preconditioners: jacobi / jacobi_apply (jacobi.hpp),
+ilu0 / milu0 / ilu_apply (ilu.hpp) and the triangular sweeps
+(triangular_solve.hpp) against the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def dv(spk, a):
+    return spk.DeviceVector.from_numpy(np.asarray(a))
+
+
+def test_jacobi_known_values(O, spk):
+    # jacobi_test.cpp:50-95
+    A = O.make_matrix(4, 4, [0, 1, 2, 3], [0, 1, 2, 3], [2., 4., 5., 10.])
+    J = spk.jacobi(spk.CsrMatrix.from_oracle(A))
+    z = spk.DeviceVector.zeros(4)
+    spk.jacobi_apply(J, dv(spk, [6., 12., 20., 30.]), z)
+    assert z.numpy().tolist() == [3.0, 3.0, 4.0, 3.0]
+
+
+def test_jacobi_bit_exact(O, spk):
+    A = O.convdiff_3d(11, 9, 7, 1.0, 0.37, 0.11, 0.05)
+    J = spk.jacobi(spk.CsrMatrix.from_oracle(A))
+    r = O.uniform_pm1(9, A.nrows)
+    z = spk.DeviceVector.zeros(A.nrows)
+    J.apply(dv(spk, r), z)
+    assert np.array_equal(z.numpy(), O.jacobi_apply(O.jacobi(A), r))
+
+
+def test_jacobi_zero_diagonal_throws(O, spk):
+    # jacobi_test.cpp:273-282
+    A = O.make_matrix(3, 3, [0, 0, 1, 2], [0, 1, 0, 2], [2., 1., 1., 3.])
+    with pytest.raises(spk.InvalidArgument):
+        spk.jacobi(spk.CsrMatrix.from_oracle(A))
+
+
+def _factor_cases(O):
+    return [
+        O.tridiagonal_matrix(9, -1.0, 4.0, -1.0),
+        O.nonsymmetric_sample(),
+        O.convdiff_centered_2d(12, 9, 1.0, 0.3, 0.2, 5.0),
+        O.convection_diffusion_2d(17, 13, 1.0, 0.45, 0.2),
+        O.convdiff_3d(7, 6, 5, 1.0, 0.4, 0.2, 0.1),
+        O.poisson_2d(10, 10, 0.5),
+    ]
+
+
+@pytest.mark.parametrize("modified", [False, True])
+def test_ilu0_factors_bit_exact(O, spk, modified):
+    """The level-scheduled device factorization performs the reference's
+    IKJ updates in the same order with no fused multiply-add: L and U match
+    bit for bit (patterns and values)."""
+    for A in _factor_cases(O):
+        Lr, Ur = O.ilu0(A, modified)
+        F = (spk.milu0 if modified else spk.ilu0)(spk.CsrMatrix.from_oracle(A))
+        (lrp, lci, lv), (urp, uci, uv) = F.factors()
+        assert np.array_equal(lrp, Lr.row_ptr) and np.array_equal(lci, Lr.col_ind)
+        assert np.array_equal(urp, Ur.row_ptr) and np.array_equal(uci, Ur.col_ind)
+        assert np.array_equal(lv, Lr.vals)
+        assert np.array_equal(uv, Ur.vals)
+
+
+def test_ilu0_apply_bit_exact(O, spk):
+    for A in _factor_cases(O):
+        Lr, Ur = O.ilu0(A)
+        F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+        r = O.uniform_pm1(11, A.nrows)
+        z = spk.DeviceVector.zeros(A.nrows)
+        spk.ilu_apply(F, dv(spk, r), z)
+        z_ref = O.backward_solve(Ur, O.forward_solve(Lr, r))
+        assert np.array_equal(z.numpy(), z_ref)
+
+
+def test_ilu0_tridiagonal_exact_solve(O, spk):
+    # ilu_test.cpp:93-107: no fill, so ilu_apply is an exact solve
+    A = O.tridiagonal_matrix(4, -1.0, 4.0, -1.0)
+    b = O.spmv(A, np.array([1., 2., 3., 4.]))
+    F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+    z = spk.DeviceVector.zeros(4)
+    spk.ilu_apply(F, dv(spk, b), z)
+    assert np.allclose(z.numpy(), [1., 2., 3., 4.], atol=1e-12, rtol=0)
+
+
+def test_ilu0_errors(O, spk):
+    # ilu_test.cpp:274-282 (non-square); ilu.hpp:63-65 (missing diagonal)
+    R = O.make_matrix(3, 4, [0, 1, 2], [0, 1, 2], [1., 2., 3.])
+    with pytest.raises(spk.InvalidArgument):
+        spk.ilu0(spk.CsrMatrix.from_oracle(R))
+    M = O.make_matrix(3, 3, [0, 1, 1, 2], [0, 0, 2, 2], [1., 1., 1., 1.])
+    with pytest.raises(spk.InvalidArgument):
+        spk.ilu0(spk.CsrMatrix.from_oracle(M))
+
+
+def test_ilu0_fp32_apply(O, spk):
+    A = O.convdiff_3d(9, 8, 7, 1.0, 0.5, 0.25, 0.125)
+    Lr, Ur = O.ilu0(A)
+    F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+    r = O.uniform_pm1(12, A.nrows)
+    z = spk.DeviceVector.zeros(A.nrows, np.float32)
+    F.apply(dv(spk, r.astype(np.float32)), z)
+    z_ref = O.backward_solve(Ur, O.forward_solve(Lr, r))
+    assert np.linalg.norm(z.numpy() - z_ref) <= 1e-5 * np.linalg.norm(z_ref)
+
+
+def test_ilu0_levels(O, spk):
+    # natural-order 7-point stencil: i+j+k hyperplanes
+    F = spk.ilu0(spk.CsrMatrix.from_oracle(O.convdiff_3d(6, 5, 4, 1., .1, .1, .1)))
+    assert F.levels() == (6 + 5 + 4 - 2, 6 + 5 + 4 - 2)

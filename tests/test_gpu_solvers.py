@@ -1,0 +1,213 @@
+"""GPU parity of the device Krylov solvers against the oracle and the
+reference's own solver tests (conjugate_gradient_test.cpp, gmres_test.cpp,
+fgmres_test.cpp, bicgstab_test.cpp, jacobi_test.cpp, ilu_test.cpp).
+
+Iteration counts: the device reduces dot products in a different (tree)
+order than std::inner_product, so counts are compared within +-1 of the
+oracle's, and every converged solve is checked on its FP64 true residual."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def dv(spk, a, dtype=np.float64):
+    return spk.DeviceVector.from_numpy(np.asarray(a, dtype=dtype))
+
+
+def grid16(O):
+    return O.poisson_2d(4, 4, 5.0)  # make_grid_16: matgen_test.cpp:348-379
+
+
+def _problems(O):
+    x4 = np.array([1., 2., 3., 4.])
+    x16 = np.arange(1, 17, dtype=float)
+    diag = O.make_matrix(4, 4, [0, 1, 2, 3], [0, 1, 2, 3], [2., 3., 4., 5.])
+    ident = O.make_matrix(4, 4, [0, 1, 2, 3], [0, 1, 2, 3], [1., 1., 1., 1.])
+    tri = O.tridiagonal_matrix(4, -1.0, 4.0, -1.0)
+    return {
+        "identity": (ident, np.array([1., 2., 3., 4.]), 1e-12),
+        "diagonal": (diag, np.array([6., 12., 20., 30.]), 1e-12),
+        "tridiag": (tri, O.spmv(tri, x4), 1e-12),
+        "grid16": (grid16(O), O.spmv(grid16(O), x16), 1e-10),
+        "nonsym": (O.nonsymmetric_sample(),
+                   O.spmv(O.nonsymmetric_sample(), x4), 1e-12),
+        "convdiff16": (O.convdiff_centered_2d(4, 4, 1.0, 0.3, 0.3, 5.0),
+                       O.spmv(O.convdiff_centered_2d(4, 4, 1.0, 0.3, 0.3, 5.0),
+                              x16), 1e-10),
+    }
+
+
+SYM = ["identity", "diagonal", "tridiag", "grid16"]
+ALL = SYM + ["nonsym", "convdiff16"]
+
+
+def _run(spk, O, method, A, b, tol, left=None, right=None, max_it=200,
+         restart=50, collect=True, x0=None, precision=None):
+    Ad = spk.CsrMatrix.from_oracle(A)
+    prec = {"jacobi": spk.jacobi, "ilu": spk.ilu0, "milu": spk.milu0,
+            None: lambda _A: spk.identity}
+    ML, MR = prec[left](Ad), prec[right](Ad)
+    p = spk.F64 if precision is None else precision
+    dt = np.float32 if p == spk.F32 else np.float64
+    bd = dv(spk, b, dt)
+    xd = dv(spk, np.zeros(A.nrows) if x0 is None else x0, dt)
+    if method == "cg":
+        cfg = spk.CGConfig(tol, restart, max_it, collect, p)
+        s = spk.conjugate_gradient(bd, xd, cfg, Ad, ML, MR)
+        xr, sr = O.cg(A, b, x0, tol=tol, restart_iterations=restart,
+                      max_iterations=max_it, left=left, right=right)
+    elif method == "gmres":
+        cfg = spk.GmresConfig(tol, restart, max_it, collect, p)
+        s = spk.gmres(bd, xd, cfg, Ad, ML, MR)
+        xr, sr = O.gmres(A, b, x0, tol=tol, restart_dimension=restart,
+                         max_iterations=max_it, left=left, right=right)
+    elif method == "fgmres":
+        cfg = spk.GmresConfig(tol, restart, max_it, collect, p)
+        s = spk.fgmres(bd, xd, cfg, Ad, ML)
+        xr, sr = O.fgmres(A, b, x0, tol=tol, restart_dimension=restart,
+                          max_iterations=max_it, preconditioner=left)
+    else:
+        cfg = spk.BicgstabConfig(tol, max_it, collect, p)
+        s = spk.bicgstab(bd, xd, cfg, Ad, ML, MR)
+        xr, sr = O.bicgstab(A, b, x0, tol=tol, max_iterations=max_it,
+                            left=left, right=right)
+    return s, xd.numpy(), sr, xr
+
+
+@pytest.mark.parametrize("method,name", [("cg", n) for n in SYM] +
+                         [(m, n) for m in ("gmres", "fgmres", "bicgstab")
+                          for n in ALL])
+def test_unpreconditioned(O, spk, method, name):
+    A, b, tol = _problems(O)[name]
+    s, x, sr, xr = _run(spk, O, method, A, b, tol, restart=20)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 1
+    assert np.allclose(x, xr, rtol=0, atol=1e-8)
+    if name == "identity":
+        assert s.computed_iterations <= 1
+
+
+@pytest.mark.parametrize("method,left,right", [
+    ("cg", "jacobi", None), ("cg", None, "jacobi"),
+    ("gmres", "jacobi", None), ("gmres", None, "jacobi"),
+    ("gmres", "ilu", None), ("gmres", None, "ilu"), ("gmres", "milu", None),
+    ("fgmres", "jacobi", None), ("fgmres", "ilu", None),
+    ("bicgstab", "jacobi", None), ("bicgstab", None, "ilu"),
+    ("bicgstab", "ilu", "jacobi"),
+])
+@pytest.mark.parametrize("name", ["tridiag", "grid16", "convdiff16"])
+def test_preconditioned(O, spk, method, left, right, name):
+    if method == "cg" and name == "convdiff16":
+        pytest.skip("CG needs a symmetric matrix")
+    A, b, tol = _problems(O)[name]
+    s, x, sr, xr = _run(spk, O, method, A, b, tol, left, right, restart=20)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 1
+    assert np.allclose(x, xr, rtol=0, atol=1e-7)
+    if "ilu" in (left, right) and name == "tridiag":
+        # preconditioned <= unpreconditioned (ilu_test.cpp:148-152)
+        su, *_ = _run(spk, O, method, A, b, tol, restart=20)
+        assert s.computed_iterations <= su.computed_iterations
+
+
+@pytest.mark.parametrize("method", ["cg", "gmres", "fgmres", "bicgstab"])
+def test_zero_rhs(O, spk, method):
+    A = O.tridiagonal_matrix(4, -1.0, 4.0, -1.0)
+    s, x, *_ = _run(spk, O, method, A, np.zeros(4), 1e-12,
+                    x0=np.array([5., 6., 7., 8.]))
+    assert s.converged and np.all(x == 0.0)
+
+
+@pytest.mark.parametrize("method,max_it,expect", [
+    ("cg", 1, 1), ("gmres", 1, 1), ("fgmres", 1, 1), ("bicgstab", 2, 2)])
+def test_max_iterations(O, spk, method, max_it, expect):
+    # conjugate_gradient_test.cpp:377-400, gmres_test.cpp:462-485,
+    # bicgstab_test.cpp:392-414
+    A = grid16(O)
+    b = O.spmv(A, np.arange(1, 17, dtype=float))
+    s, *_ = _run(spk, O, method, A, b, 1e-14, max_it=max_it, restart=10)
+    assert not s.converged
+    assert s.computed_iterations == expect
+
+
+@pytest.mark.parametrize("method", ["gmres", "fgmres"])
+def test_residual_history_monotone(O, spk, method):
+    # gmres_test.cpp:487-521, fgmres_test.cpp:381-415
+    A = grid16(O)
+    b = O.spmv(A, np.arange(1, 17, dtype=float))
+    s, x, sr, xr = _run(spk, O, method, A, b, 1e-10, restart=20)
+    h = np.array(s.iteration_residuals)
+    assert len(h) == s.computed_iterations
+    assert np.all(h[1:] <= h[:-1] + 1e-14)
+    assert np.allclose(h, sr.iteration_residuals, rtol=1e-9, atol=1e-13)
+
+
+def test_cg_restart_semantics(O, spk):
+    """restart_iterations resets the search direction
+    (conjugate_gradient.hpp:208-230): with restart 5 the iteration count and
+    residual history follow the oracle's."""
+    A = O.poisson_2d(24, 24, 0.05)
+    b = O.uniform_pm1(3, A.nrows)
+    s, x, sr, xr = _run(spk, O, "cg", A, b, 1e-9, left="jacobi", restart=5,
+                        max_it=5000)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 2
+    k = min(50, len(s.iteration_residuals), len(sr.iteration_residuals))
+    assert np.allclose(s.iteration_residuals[:k], sr.iteration_residuals[:k],
+                       rtol=1e-6)
+
+
+def test_bicgstab_breakdown_reported(O, spk):
+    """x0 exact: rho_new = <r^, r> = 0.  The reference returns from the step
+    without consuming budget and loops forever (bicgstab.hpp:203-204); the
+    device solver stops and reports the breakdown."""
+    A = O.tridiagonal_matrix(6, -1.0, 4.0, -1.0)
+    xt = np.arange(1, 7, dtype=float)
+    s, x, *_ = _run(spk, O, "bicgstab", A, O.spmv(A, xt), 1e-12, x0=xt)
+    assert s.breakdown and not s.converged
+    assert np.array_equal(x, xt)
+
+
+def test_gmres_exact_start(O, spk):
+    """beta = 0 at a restart (x0 exact) is reported as converged instead of
+    the reference's division by zero (gmres.hpp:229-231)."""
+    A = O.nonsymmetric_sample()
+    xt = np.array([1., 2., 3., 4.])
+    s, x, *_ = _run(spk, O, "gmres", A, O.spmv(A, xt), 1e-12, x0=xt)
+    assert s.converged and np.array_equal(x, xt)
+
+
+@pytest.mark.parametrize("method,left", [("cg", "jacobi"), ("gmres", "ilu"),
+                                          ("bicgstab", "ilu")])
+def test_medium_problems(O, spk, method, left):
+    if method == "cg":
+        A = O.poisson_3d(24, 20, 16)
+    else:
+        A = O.convdiff_3d(20, 16, 12, 1.0, 0.4, 0.2, 0.1)
+    b = O.uniform_pm1(42, A.nrows)
+    s, x, sr, xr = _run(spk, O, method, A, b, 1e-8, left=left, restart=30,
+                        max_it=3000)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= max(
+        2, sr.computed_iterations // 50)
+    # gmres with a left preconditioner tests the preconditioned residual
+    # (gmres.hpp:202); the true residual is then checked against the
+    # oracle's own solution
+    rel = O.true_relres(A, b, x)
+    rel_ref = O.true_relres(A, b, xr)
+    assert rel <= max(1e-8, 10 * rel_ref)
+
+
+@pytest.mark.parametrize("method", ["cg", "gmres", "bicgstab"])
+def test_fp32_vectors(O, spk, method):
+    """precision = SPK_F32: FP32 vectors/SpMV, FP64 dots and scalars."""
+    if method == "cg":
+        A, left = O.poisson_3d(16, 16, 16), "jacobi"
+    else:
+        A, left = O.convdiff_3d(16, 12, 10, 1.0, 0.4, 0.2, 0.1), "ilu"
+    b = O.uniform_pm1(5, A.nrows)
+    s, x, sr, xr = _run(spk, O, method, A, b, 1e-5, left=left, restart=30,
+                        max_it=2000, precision=spk.F32)
+    assert s.converged
+    assert O.true_relres(A, b, x.astype(np.float64)) < 1e-4

@@ -1,0 +1,136 @@
+"""GPU parity of the device store, SpMV, diagonal extraction and generators
+against the oracle (sparse_blas.hpp:17-34, unary.hpp:18-39,
+Compressed_row_sparsity_Impl.cpp:10-40, matgen.hpp)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_csr(O, rng, nrows, ncols, lengths):
+    rows, cols = [], []
+    for r, L in enumerate(lengths):
+        c = rng.choice(ncols, size=min(L, ncols), replace=False)
+        rows += [r] * len(c)
+        cols += list(c)
+    vals = rng.uniform(-1, 1, len(rows))
+    return O.make_matrix(nrows, ncols, rows, cols, vals)
+
+
+def _spmv_dev(spk, A, x, dtype=np.float64):
+    Ad = spk.CsrMatrix.from_oracle(A)
+    return spk.multiply(Ad, spk.DeviceVector.from_numpy(x.astype(dtype))).numpy()
+
+
+def test_spmv_known_3x3(O, spk):
+    # sparse_blas_test.cpp:36-54
+    A = O.make_matrix(3, 3, [0, 0, 1, 2, 2], [0, 2, 1, 0, 2],
+                      [2., 3., 4., 5., 6.])
+    y = _spmv_dev(spk, A, np.array([1., 2., 3.]))
+    assert y.tolist() == [11.0, 8.0, 23.0]
+
+
+def test_spmv_empty_rectangular_identity(O, spk):
+    # sparse_blas_test.cpp:56-95
+    E = O.make_matrix(3, 3, [], [], [])
+    assert _spmv_dev(spk, E, np.array([1., 2., 3.])).tolist() == [0., 0., 0.]
+    R = O.make_matrix(2, 3, [0, 0, 1], [0, 2, 1], [1., 2., 3.])
+    assert _spmv_dev(spk, R, np.array([1., 2., 3.])).tolist() == [7., 6.]
+    I = O.make_matrix(3, 3, [0, 1, 2], [0, 1, 2], [1., 1., 1.])
+    assert _spmv_dev(spk, I, np.array([4., 5., 6.])).tolist() == [4., 5., 6.]
+
+
+def test_spmv_short_rows_bit_exact(O, spk):
+    """Rows summed by one lane reproduce the reference's sequential sum
+    bit for bit."""
+    A = O.poisson_2d(96, 80, 0.25)
+    x = O.uniform_pm1(3, A.ncols)
+    assert np.array_equal(_spmv_dev(spk, A, x), O.spmv(A, x))
+    B = O.convdiff_3d(20, 17, 13, 1.0, 0.3, 0.2, 0.1)
+    xb = O.uniform_pm1(4, B.ncols)
+    assert np.array_equal(_spmv_dev(spk, B, xb), O.spmv(B, xb))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_spmv_mixed_row_lengths(O, spk, seed):
+    """Empty rows, short rows, medium rows (several lanes per row) and rows
+    longer than a block's nonzero budget (block-wide reduction)."""
+    rng = np.random.default_rng(seed)
+    n = 3000
+    lengths = rng.choice([0, 1, 3, 7, 40, 300, 1500, 2500, 6000], size=n,
+                         p=[.05, .15, .3, .3, .1, .06, .02, .01, .01])
+    A = _random_csr(O, rng, n, 7000, lengths)
+    x = rng.uniform(-1, 1, A.ncols)
+    y = _spmv_dev(spk, A, x)
+    y_ref = O.spmv(A, x)
+    absrow = np.abs(A.to_scipy()) @ np.abs(x)
+    assert np.all(np.abs(y - y_ref) <= 1e-12 * absrow + 1e-300)
+    assert np.linalg.norm(y - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    nb, nl = Ad.plan_info()
+    assert nl == int(np.sum(lengths > 2048))
+
+
+def test_spmv_fp32(O, spk):
+    rng = np.random.default_rng(5)
+    lengths = rng.integers(0, 60, 5000)
+    A = _random_csr(O, rng, 5000, 5000, lengths)
+    x = rng.uniform(-1, 1, A.ncols).astype(np.float32)
+    y = _spmv_dev(spk, A, x, np.float32)
+    y_ref = O.spmv(A, x.astype(np.float64))
+    absrow = np.abs(A.to_scipy()) @ np.abs(x.astype(np.float64))
+    assert np.all(np.abs(y - y_ref) <= 2e-6 * absrow + 1e-30)
+
+
+def test_store_round_trip(O, spk):
+    A = O.convection_diffusion_2d(30, 20, 1.0, 0.4, 0.1)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    rp, ci, v = Ad.download()
+    assert np.array_equal(rp, A.row_ptr)
+    assert np.array_equal(ci, A.col_ind)
+    assert np.array_equal(v, A.vals)
+
+
+def test_store_rejects_bad_pattern(spk):
+    with pytest.raises(spk.InvalidArgument):
+        spk.CsrMatrix.from_host(2, 2, [0, 1, 2], [0, 5], [1.0, 1.0])
+    with pytest.raises(spk.InvalidArgument):
+        spk.CsrMatrix.from_host(2, 2, [0, 2, 1], [0, 1], [1.0, 1.0])
+
+
+def test_extract_diagonal(O, spk):
+    # unary_test.cpp:35-95: known values, rectangular, missing entries
+    A = O.make_matrix(3, 4, [0, 0, 1, 2, 2], [0, 3, 2, 0, 2],
+                      [5., 1., 7., 2., 9.])
+    d = spk.extract_diagonal(spk.CsrMatrix.from_oracle(A)).numpy()
+    assert d.tolist() == O.extract_diagonal(A).tolist() == [5.0, 0.0, 9.0]
+
+
+def test_generators_bit_exact(O, spk):
+    cases = [
+        (spk.poisson_2d(33, 17, 0.0), O.poisson_2d(33, 17, 0.0)),
+        (spk.poisson_2d(16, 16, 5.0), O.poisson_2d(16, 16, 5.0)),
+        (spk.poisson_3d(9, 7, 5), O.poisson_3d(9, 7, 5)),
+        (spk.convdiff_3d(8, 6, 5, 1.0, 0.37, 0.18, 0.09),
+         O.convdiff_3d(8, 6, 5, 1.0, 0.37, 0.18, 0.09)),
+    ]
+    for Ad, A in cases:
+        rp, ci, v = Ad.download()
+        assert np.array_equal(rp, A.row_ptr)
+        assert np.array_equal(ci, A.col_ind)
+        assert np.array_equal(v, A.vals)
+
+
+def test_uniform_rhs_bit_exact(O, spk):
+    assert np.array_equal(spk.uniform_pm1(42, 10001).numpy(),
+                          O.uniform_pm1(42, 10001))
+
+
+def test_true_relres(O, spk):
+    A = O.poisson_2d(40, 40, 1.0)
+    b = O.uniform_pm1(1, A.nrows)
+    x = O.uniform_pm1(2, A.nrows)
+    r = spk.true_relres(spk.CsrMatrix.from_oracle(A),
+                        spk.DeviceVector.from_numpy(b),
+                        spk.DeviceVector.from_numpy(x))
+    assert abs(r - O.true_relres(A, b, x)) <= 1e-13 * r
