@@ -126,6 +126,20 @@ def main():
     log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s "
         f"({spmv_gbs / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
+    # device-to-device copy of the same size as one SpMV's traffic: the
+    # practical streaming ceiling next to the 8 TB/s peak
+    cbytes = (bs // 2) // 8 * 8
+    ca, cb = spk.DeviceVector(cbytes // 8), spk.DeviceVector(cbytes // 8)
+    spk._lib.call("spk_memset", ca.ptr, 0, cbytes)
+    for _ in range(3):
+        spk._lib.call("spk_memcpy_dtod", cb.ptr, ca.ptr, cbytes)
+    timer.start()
+    for _ in range(20):
+        spk._lib.call("spk_memcpy_dtod", cb.ptr, ca.ptr, cbytes)
+    copy_ms = timer.stop() / 20
+    copy_gbs = 2 * cbytes / (copy_ms * 1e-3) / 1e9
+    log(f"copy: {copy_gbs:.0f} GB/s (read+write)")
+
     out = {
         "metric": "time-to-1e-8 residual: learned-FGMRES vs Jacobi-PCG; "
                   "SpMV GB/s % HBM peak",
@@ -150,7 +164,9 @@ def main():
         "spmv": {"us_per_launch": round(spmv_ms * 1e3, 2),
                  "GBs": round(spmv_gbs, 1),
                  "frac_hbm_peak": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                 "bytes_per_launch": bs},
+                 "bytes_per_launch": bs,
+                 "frac_of_copy": round(spmv_gbs / copy_gbs, 4)},
+        "copy_GBs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),

@@ -113,7 +113,6 @@ spk_csr_s* build_generated(int64_t n, LenK len_kernel, FillK fill) {
   hipStream_t s = stream();
   auto* A = new spk_csr_s();
   try {
-    A->nrows = A->ncols = n;
     DevBuf<int32_t> len((size_t)n);
     unsigned g = (unsigned)((n + 255) / 256);
     len_kernel(g, len.p);
@@ -128,10 +127,7 @@ spk_csr_s* build_generated(int64_t n, LenK len_kernel, FillK fill) {
       SPK_REQUIRE(acc < INT32_MAX, SPK_INVALID_ARGUMENT, "nnz exceeds int32");
       rp[r + 1] = (int32_t)acc;
     }
-    A->nnz = acc;
-    A->rp.alloc((size_t)n + 1);
-    A->ci.alloc((size_t)acc);
-    A->v64.alloc((size_t)acc);
+    A->allocate(n, n, acc);
     SPK_HIP_TRY(hipMemcpyAsync(A->rp.p, rp.data(), (n + 1) * 4,
                                hipMemcpyHostToDevice, s));
     fill(g, A->rp.p, A->ci.p, A->v64.p);

@@ -110,10 +110,23 @@ void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks) {
 
 }  // namespace spk
 
+void spk_csr_s::allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_) {
+  nrows = nrows_;
+  ncols = ncols_;
+  nnz = nnz_;
+  rp.alloc((size_t)nrows + 1);
+  ci.alloc((size_t)nnz + spk::kPad);
+  v64.alloc((size_t)nnz + spk::kPad);
+  hipStream_t s = spk::stream();
+  SPK_HIP_TRY(hipMemsetAsync(ci.p + nnz, 0, spk::kPad * sizeof(int32_t), s));
+  SPK_HIP_TRY(hipMemsetAsync(v64.p + nnz, 0, spk::kPad * sizeof(double), s));
+}
+
 const float* spk_csr_s::vals32() {
   if (!has32) {
-    v32.alloc((size_t)nnz);
+    v32.alloc((size_t)nnz + spk::kPad);
     hipStream_t s = spk::stream();
+    SPK_HIP_TRY(hipMemsetAsync(v32.p + nnz, 0, spk::kPad * sizeof(float), s));
     if (nnz > 0)
       hipLaunchKernelGGL(spk::f64_to_f32_kernel, dim3(spk::grid_for(nnz)),
                          dim3(256), 0, s, v64.p, v32.p, nnz);
@@ -149,12 +162,7 @@ spk_status spk_csr_create(int64_t nrows, int64_t ncols, int64_t nnz,
     }
     auto* A = new spk_csr_s();
     try {
-      A->nrows = nrows;
-      A->ncols = ncols;
-      A->nnz = nnz;
-      A->rp.alloc((size_t)nrows + 1);
-      A->ci.alloc((size_t)nnz);
-      A->v64.alloc((size_t)nnz);
+      A->allocate(nrows, ncols, nnz);
       hipStream_t s = stream();
       SPK_HIP_TRY(hipMemcpyAsync(A->rp.p, rp32.data(), rp32.size() * 4,
                                  hipMemcpyHostToDevice, s));
@@ -183,12 +191,7 @@ spk_status spk_csr_create_device(int64_t nrows, int64_t ncols, int64_t nnz,
                     nnz < INT_MAX, SPK_INVALID_ARGUMENT, "bad size");
     auto* A = new spk_csr_s();
     try {
-      A->nrows = nrows;
-      A->ncols = ncols;
-      A->nnz = nnz;
-      A->rp.alloc((size_t)nrows + 1);
-      A->ci.alloc((size_t)nnz);
-      A->v64.alloc((size_t)nnz);
+      A->allocate(nrows, ncols, nnz);
       hipStream_t s = stream();
       SPK_HIP_TRY(hipMemcpyAsync(A->rp.p, d_row_ptr, (nrows + 1) * 4,
                                  hipMemcpyDeviceToDevice, s));

@@ -2,22 +2,30 @@
 // SpMV plan and the SpMV kernel template shared by the solvers.
 //
 // Layout in HBM (one store):
-//   rp[nrows+1]  int32   row offsets      (Compressed_row_sparsity::row_ptr)
-//   ci[nnz]      int32   column indices   (Compressed_row_sparsity::col_ind)
-//   v64[nnz]     double  values           (Compressed_row_matrix::values)
-//   v32[nnz]     float   FP32 copy, made on first FP32 use
-//   plan[nb+1]   int2    {first row, first nonzero} of each SpMV row block
+//   rp[nrows+1]      int32   row offsets      (Compressed_row_sparsity::row_ptr)
+//   ci[nnz+kPad]     int32   column indices   (Compressed_row_sparsity::col_ind)
+//   v64[nnz+kPad]    double  values           (Compressed_row_matrix::values)
+//   v32[nnz+kPad]    float   FP32 copy, made on first FP32 use
+//   plan[nb+1]       int2    {first row, first nonzero} of each SpMV row block
+// The kPad zeroed tail lets the kernel read index/value PAIRS from an even
+// start without leaving the allocation.
 //
 // Plan: rows are cut greedily into blocks of at most kSpmvBlock rows and at
 // most kNnzBlock nonzeros; a row longer than kNnzBlock gets a block of its
 // own (a "long" block).  One workgroup of kSpmvBlock threads runs per block:
-//  - short/medium rows: the block's nonzeros are streamed with coalesced
-//    loads of ci/v, multiplied by the gathered x and staged in LDS; then each
-//    row is summed from LDS by L lanes (L = largest power of two with
-//    rows * L <= kSpmvBlock, at most 64).  With L = 1 a lane adds its row's
-//    products in storage order, which is sparse_blas.hpp:26-31's order, so
-//    FP64 results are bit-identical to the reference for such rows.
+//  - short/medium rows: the block's nonzeros are streamed with 16-byte
+//    (pairwise) nontemporal loads of ci and v -- the matrix is read once per
+//    SpMV, so it bypasses the caches that keep x --, multiplied by the
+//    gathered x and staged in LDS; then each row is summed from LDS by L
+//    lanes (L: power of two, at most 64, L * rows <= kSpmvBlock, about eight
+//    products per lane).  With L = 1 a lane adds its row's products in
+//    storage order, which is sparse_blas.hpp:26-31's order, so FP64 results
+//    are bit-identical to the reference for such rows (all stencil rows).
 //  - long rows: all threads stride over the row, then a block reduction.
+// Blocks are dispatched in plan order: measured on MI355X, keeping
+// consecutive row blocks on different XCDs (the hardware's round-robin) is
+// faster than remapping contiguous ranges onto one XCD (the x working set
+// of a 7-point stencil is then shared through the Infinity Cache).
 #pragma once
 
 #include "spk_internal.hpp"
@@ -25,8 +33,9 @@
 
 namespace spk {
 
-constexpr int kSpmvBlock = 256;
-constexpr int kNnzBlock = 2048;
+constexpr int kSpmvBlock = 512;
+constexpr int kNnzBlock = 4096;
+constexpr int kPad = 4;  // zeroed tail elements of ci / v
 
 template <typename T>
 struct CsrView {
@@ -49,6 +58,8 @@ struct spk_csr_s {
   int nblocks = 0, nlong = 0;
   std::vector<int32_t> h_rp;  // host copy of the row offsets
 
+  // allocates rp, ci and v64 (with the zeroed kPad tail)
+  void allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_);
   const float* vals32();
   template <typename T>
   spk::CsrView<T> view();
@@ -110,15 +121,26 @@ struct EpiFinalize {  // adapts Epi::finalize to the finisher's fin(totals)
 
 // ------------------------------------------------------------- kernel
 
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+typedef int IdxPair __attribute__((ext_vector_type(2)));
+
 template <typename T, typename Epi>
 __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
-    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials,
-    int nxcd) {
-  __shared__ T prod[kNnzBlock];
+    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials) {
+  __shared__ T prod[kNnzBlock + 2];
   __shared__ double red[kSpmvBlock / 64];
   if (epi.done && *epi.done) return;
   const int tid = threadIdx.x;
-  const int b = xcd_remap(blockIdx.x, gridDim.x, nxcd);
+  const int b = blockIdx.x;
   const int2 p0 = A.plan[b], p1 = A.plan[b + 1];
   const int r0 = p0.x, base = p0.y, nrows = p1.x - p0.x, cnt = p1.y - p0.y;
   double acc = 0.0;
@@ -126,40 +148,50 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     // long row: the block holds exactly one row
     T s = T(0);
     for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
-      s += A.v[k] * x[A.ci[k]];
+      s += __builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
     s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
     if (tid == 0) acc = epi.row(r0, s);
   } else {
-    // lanes per row: enough rows to occupy the block, at least ~8 products
+    // lanes per row: enough rows to occupy the block, about eight products
     // per lane; short rows (stencils) get one lane each
     int L = 1;
     while (L < 64 && 2 * L * nrows <= kSpmvBlock && 16 * L * nrows <= cnt) L *= 2;
-    constexpr int U = kNnzBlock / kSpmvBlock;
-    // issue every index/value load of the block, and the row bounds, before
-    // the gathers: three dependent memory trips per block in total
-    int c[U];
-    T a[U];
+    // pairs of (index, value) from the even-aligned start k0 <= base; every
+    // load of the block and the row bounds are issued before the gathers
+    const int k0 = base & ~1, off = base - k0, span = cnt + off;
+    constexpr int U = (kNnzBlock + 2) / (2 * kSpmvBlock) + 1;
+    typedef typename Pair<T>::type TP;
+    IdxPair c[U];
+    TP a[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = tid + u * kSpmvBlock;
-      const bool ok = k < cnt;
-      c[u] = ok ? A.ci[base + k] : 0;
-      a[u] = ok ? A.v[base + k] : T(0);
+      const int k = 2 * tid + 2 * u * kSpmvBlock;
+      if (k < span) {
+        c[u] = __builtin_nontemporal_load((const IdxPair*)(A.ci + k0 + k));
+        a[u] = __builtin_nontemporal_load((const TP*)(A.v + k0 + k));
+      } else {
+        c[u] = IdxPair{0, 0};
+        a[u] = TP{T(0), T(0)};
+      }
     }
     const int g = tid / L, lane = tid % L;
     int s0 = 0, s1 = 0;
     if (g < nrows) {
-      s0 = A.rp[r0 + g] - base;
-      s1 = A.rp[r0 + g + 1] - base;
+      s0 = A.rp[r0 + g] - base + off;
+      s1 = A.rp[r0 + g + 1] - base + off;
     }
-    T xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      xv[u] = (tid + u * kSpmvBlock < cnt) ? x[c[u]] : T(0);
+    T xv0[U], xv1[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = tid + u * kSpmvBlock;
-      if (k < cnt) prod[k] = a[u] * xv[u];
+      const int k = 2 * tid + 2 * u * kSpmvBlock;
+      xv0[u] = (k < span) ? x[c[u].x] : T(0);
+      xv1[u] = (k + 1 < span) ? x[c[u].y] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 2 * tid + 2 * u * kSpmvBlock;
+      if (k < span) prod[k] = a[u].x * xv0[u];
+      if (k + 1 < span) prod[k + 1] = a[u].y * xv1[u];
     }
     __syncthreads();
     if (L == 1) {
@@ -173,7 +205,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
       T s = T(0);
       if (g < nrows)
         for (int j = s0 + lane; j < s1; j += L) s += prod[j];
-      for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
       if (g < nrows && lane == 0) acc = epi.row(r0 + g, s);
     }
   }
@@ -188,7 +220,7 @@ void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
   if (A.nrows == 0) return;
   double* parts = Epi::kReduce ? reduce_partials(A.nblocks) : nullptr;
   hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
-                     0, s, A.view<T>(), x, epi, parts, ctx().num_xcds);
+                     0, s, A.view<T>(), x, epi, parts);
   SPK_HIP_TRY(hipGetLastError());
   if constexpr (Epi::kReduce) {
     hipLaunchKernelGGL((finish_kernel<1, EpiFinalize<Epi>>), dim3(1),

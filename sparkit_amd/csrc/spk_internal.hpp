@@ -94,7 +94,6 @@ struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
   int num_cus = 0;
-  int num_xcds = 8;
 };
 
 Context& ctx();             // initialises device 0 on first use
@@ -108,16 +107,6 @@ double* reduce_partials(size_t count);
 // ------------------------------------------------------------- launch
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
-
-// Workgroups are dispatched round-robin over the XCDs (hardware id w lands
-// on XCD w % nxcd).  Remapping so that each XCD walks a contiguous range of
-// logical blocks keeps neighbouring row blocks -- and the x entries they
-// share -- in one XCD's L2.  The map is a bijection on [0, G).
-__device__ __forceinline__ int xcd_remap(int w, int G, int nxcd) {
-  int q = G / nxcd, rem = G % nxcd;
-  int xcd = w % nxcd, idx = w / nxcd;
-  return xcd < rem ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
-}
 
 // ------------------------------------------------------------- reductions
 

@@ -136,11 +136,7 @@ std::vector<int32_t> group_levels(const std::vector<int32_t>& lvl,
 spk_csr_s* make_store(int64_t n, std::vector<int32_t>&& rp,
                       const std::vector<int32_t>& ci) {
   auto* S = new spk_csr_s();
-  S->nrows = S->ncols = n;
-  S->nnz = rp[n];
-  S->rp.alloc((size_t)n + 1);
-  S->ci.alloc((size_t)S->nnz);
-  S->v64.alloc((size_t)S->nnz);
+  S->allocate(n, n, rp[n]);
   hipStream_t s = stream();
   SPK_HIP_TRY(hipMemcpyAsync(S->rp.p, rp.data(), rp.size() * 4,
                              hipMemcpyHostToDevice, s));

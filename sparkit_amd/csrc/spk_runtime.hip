@@ -28,8 +28,6 @@ void init_device(int device) {
   SPK_HIP_TRY(hipGetDeviceProperties(&prop, device));
   g_ctx.device = device;
   g_ctx.num_cus = prop.multiProcessorCount;
-  // MI355X: 256 CUs in 8 XCDs of 32.
-  g_ctx.num_xcds = prop.multiProcessorCount >= 64 ? 8 : 1;
   SPK_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   SPK_HIP_TRY(hipEventCreate(&g_ev0));
   SPK_HIP_TRY(hipEventCreate(&g_ev1));

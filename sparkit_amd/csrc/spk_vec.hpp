@@ -67,11 +67,18 @@ __global__ __launch_bounds__(kFinishBlock) void finish_kernel(
   if (done && *done) return;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += kFinishBlock)
-      s += partials[(size_t)i * W + k];
-    s = block_reduce_sum<double, kFinishBlock>(s, red);
-    if (threadIdx.x == 0) tot[k] = s;
+    // eight independent accumulators keep eight loads in flight per thread;
+    // the order of the additions depends only on count
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int i = threadIdx.x;
+    for (; i + 7 * kFinishBlock < count; i += 8 * kFinishBlock) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s[u] += partials[(size_t)(i + u * kFinishBlock) * W + k];
+    }
+    for (; i < count; i += kFinishBlock) s[0] += partials[(size_t)i * W + k];
+    double t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    t = block_reduce_sum<double, kFinishBlock>(t, red);
+    if (threadIdx.x == 0) tot[k] = t;
   }
   if (threadIdx.x == 0) fin((const double*)tot);
 }

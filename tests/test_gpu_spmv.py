@@ -57,7 +57,7 @@ def test_spmv_mixed_row_lengths(O, spk, seed):
     longer than a block's nonzero budget (block-wide reduction)."""
     rng = np.random.default_rng(seed)
     n = 3000
-    lengths = rng.choice([0, 1, 3, 7, 40, 300, 1500, 2500, 6000], size=n,
+    lengths = rng.choice([0, 1, 3, 7, 40, 300, 1500, 4500, 6000], size=n,
                          p=[.05, .15, .3, .3, .1, .06, .02, .01, .01])
     A = _random_csr(O, rng, n, 7000, lengths)
     x = rng.uniform(-1, 1, A.ncols)
@@ -68,7 +68,7 @@ def test_spmv_mixed_row_lengths(O, spk, seed):
     assert np.linalg.norm(y - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
     Ad = spk.CsrMatrix.from_oracle(A)
     nb, nl = Ad.plan_info()
-    assert nl == int(np.sum(lengths > 2048))
+    assert nl == int(np.sum(lengths > 4096))
 
 
 def test_spmv_fp32(O, spk):
