@@ -145,8 +145,9 @@ spk_status spk_csr_create(int64_t nrows, int64_t ncols, int64_t nnz,
                           const double* vals, spk_csr* out) {
   return guarded([&] {
     SPK_REQUIRE(out && row_ptr, SPK_INVALID_ARGUMENT, "null argument");
-    SPK_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, SPK_INVALID_ARGUMENT,
-                "negative size");
+    // Shape(r, c) throws std::logic_error for r <= 1 or c <= 1 (Shape.cpp:8-10)
+    SPK_REQUIRE(nrows > 1 && ncols > 1, SPK_LOGIC_ERROR, "invalid matrix shape");
+    SPK_REQUIRE(nnz >= 0, SPK_INVALID_ARGUMENT, "negative size");
     SPK_REQUIRE(nrows < INT_MAX && nnz < INT_MAX, SPK_INVALID_ARGUMENT,
                 "nrows and nnz must fit in int32 for the device store");
     SPK_REQUIRE(nnz == 0 || (col_ind && vals), SPK_INVALID_ARGUMENT,
@@ -187,8 +188,9 @@ spk_status spk_csr_create_device(int64_t nrows, int64_t ncols, int64_t nnz,
                                  const double* d_vals, spk_csr* out) {
   return guarded([&] {
     SPK_REQUIRE(out && d_row_ptr, SPK_INVALID_ARGUMENT, "null argument");
-    SPK_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0 && nrows < INT_MAX &&
-                    nnz < INT_MAX, SPK_INVALID_ARGUMENT, "bad size");
+    SPK_REQUIRE(nrows > 1 && ncols > 1, SPK_LOGIC_ERROR, "invalid matrix shape");
+    SPK_REQUIRE(nnz >= 0 && nrows < INT_MAX && nnz < INT_MAX,
+                SPK_INVALID_ARGUMENT, "bad size");
     auto* A = new spk_csr_s();
     try {
       A->allocate(nrows, ncols, nnz);

@@ -96,6 +96,9 @@ def test_store_rejects_bad_pattern(spk):
         spk.CsrMatrix.from_host(2, 2, [0, 1, 2], [0, 5], [1.0, 1.0])
     with pytest.raises(spk.InvalidArgument):
         spk.CsrMatrix.from_host(2, 2, [0, 2, 1], [0, 1], [1.0, 1.0])
+    # Shape(r, c) with r <= 1 or c <= 1 is a logic_error (Shape.cpp:8-10)
+    with pytest.raises(spk.LogicError):
+        spk.CsrMatrix.from_host(1, 3, [0, 1], [0], [1.0])
 
 
 def test_extract_diagonal(O, spk):
