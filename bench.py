@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark: FP64 CSR SpMV + Jacobi-PCG on one MI355X.
 
-Workload (the configuration the metric is quoted on): 3D 7-point Poisson,
-192^3 grid (7,077,888 rows, 49,324,032 nnz, int32 indices, Dirichlet
-diagonal 6), RHS splitmix64(seed 42) uniform[-1,1], x0 = 0, relative
-residual tolerance 1e-8, Jacobi left preconditioner, plain PCG
-(restart_iterations = max_iterations).  One step = one full PCG solve to
-1e-8 with the matrix and vectors already in HBM.
+Workload (the configuration the metric is quoted on), two inputs:
+  (a) 3D 7-point Poisson, 192^3 grid: 7,077,888 rows, 49,324,032 nnz,
+      Dirichlet diagonal 6;
+  (b) power-law SPD, 4,194,304 rows: Zipf(1.5) row lengths on [3, 4096],
+      symmetrised, values uniform[-1,0), diagonal = sum|off| + 1;
+both with int32 indices, RHS splitmix64(seed 42) uniform[-1,1], x0 = 0,
+relative residual tolerance 1e-8, Jacobi left preconditioner, plain PCG
+(restart_iterations = max_iterations).  One step = one full PCG solve to 1e-8
+of each input, matrices and vectors already in HBM.
 
-value: algorithmic HBM GB/s sustained over the whole solve
-       (iterations x B_pcg / solve time, B_pcg = B_spmv + 11*8*n).
-roofline: the SpMV kernel (the dominant kernel), B_spmv per launch / its
-       average duration measured with HIP events on the library stream.
-cpu_baseline: the oracle's Jacobi-PCG (the CPU restatement of the
-       reference) on the same matrix for a bounded number of iterations,
-       one core.
+value: algorithmic HBM GB/s sustained over the step
+       (sum of iterations x B_pcg over both solves / step time;
+       B_pcg = B_spmv + 11*8*n, B_spmv = nnz*12 + 4(n+1) + 16n).
+roofline: the SpMV kernel of the input whose SpMV takes the larger share of
+       the step, B_spmv per launch / its average duration measured with HIP
+       events on the library stream (the stream every kernel runs on).
+cpu_baseline: the oracle's Jacobi-PCG (CPU restatement of the reference) on
+       input (a) for a bounded number of iterations, one core.
 """
 import argparse
 import json
@@ -28,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = ("time-to-1e-8 residual: learned-FGMRES vs Jacobi-PCG; "
+          "SpMV GB/s % HBM peak")
 
 
 def log(*a):
@@ -38,8 +44,12 @@ def b_spmv(n, nnz, s=8):
     return nnz * (s + 4) + 4 * (n + 1) + 2 * s * n
 
 
+def b_pcg(n, nnz, s=8):
+    return b_spmv(n, nnz, s) + 11 * s * n
+
+
 def cpu_baseline(grid, iters):
-    """Oracle Jacobi-PCG, `iters` iterations on the same matrix (1 core)."""
+    """Oracle Jacobi-PCG, `iters` iterations on input (a) (1 core)."""
     from oracle import oracle as O
     A = O.poisson_3d(grid)
     b = O.uniform_pm1(42, A.nrows)
@@ -48,13 +58,41 @@ def cpu_baseline(grid, iters):
                 max_iterations=iters, left="jacobi")
     dt = time.perf_counter() - t0
     n, nnz = A.nrows, A.nnz
-    bytes_ = s.computed_iterations * (b_spmv(n, nnz) + 11 * 8 * n)
+    bytes_ = s.computed_iterations * b_pcg(n, nnz)
     return {"value": round(bytes_ / dt / 1e9, 3), "unit": "GB/s",
             "cores": 1, "kind": "port",
             "sample": f"{s.computed_iterations} Jacobi-PCG iterations of the "
-                      f"oracle on the {grid}^3 Poisson matrix "
-                      f"({dt:.2f} s; {dt / s.computed_iterations * 1e3:.1f} "
-                      f"ms/iteration)"}
+                      f"oracle on the {grid}^3 Poisson matrix ({dt:.2f} s, "
+                      f"{dt / s.computed_iterations * 1e3:.1f} ms/iteration; "
+                      f"a full 644-iteration solve would take "
+                      f"{dt / s.computed_iterations * 644:.0f} s)"}
+
+
+class Input:
+    def __init__(self, spk, name, A):
+        self.name, self.A = name, A
+        self.n, self.nnz = A.nrows, A.nnz
+        self.b = spk.uniform_pm1(42, self.n)
+        self.x = spk.DeviceVector.zeros(self.n)
+        self.J = spk.jacobi(A)
+        self.cfg = spk.CGConfig(tolerance=1e-8, restart_iterations=20000,
+                                max_iterations=20000)
+
+    def solve(self, spk):
+        spk._lib.call("spk_memset", self.x.ptr, 0, self.x.nbytes)
+        return spk.conjugate_gradient(self.b, self.x, self.cfg, self.A,
+                                      self.J, spk.identity)
+
+
+def spmv_time(spk, A, x, reps):
+    timer = spk.Timer()
+    y = spk.DeviceVector(A.nrows)
+    for _ in range(3):
+        A.spmv(x, y)
+    timer.start()
+    for _ in range(reps):
+        A.spmv(x, y)
+    return timer.stop() / reps
 
 
 def main():
@@ -63,7 +101,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=192)
-    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--powerlaw-n", type=int, default=4194304)
+    ap.add_argument("--inputs", default="ab", help="a, b or ab")
+    ap.add_argument("--spmv-reps", type=int, default=30)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -77,58 +117,72 @@ def main():
     spk.init(0)
     cus, name = spk.device_info()
     log(f"device: {name}, {cus} CUs")
-    g = args.grid
-    A = spk.poisson_3d(g)
-    n, nnz = A.nrows, A.nnz
-    log(f"matrix: poisson_3d({g}) n={n} nnz={nnz}")
-    b = spk.uniform_pm1(42, n)
-    x = spk.DeviceVector.zeros(n)
-    J = spk.jacobi(A)
-    cfg = spk.CGConfig(tolerance=1e-8, restart_iterations=20000,
-                       max_iterations=20000)
+    inputs = []
+    if "a" in args.inputs:
+        t0 = time.perf_counter()
+        A = spk.poisson_3d(args.grid)
+        log(f"(a) poisson_3d({args.grid}): n={A.nrows} nnz={A.nnz} "
+            f"built in {time.perf_counter() - t0:.2f} s")
+        inputs.append(Input(spk, f"3D 7-point Poisson {args.grid}^3", A))
+    if "b" in args.inputs:
+        t0 = time.perf_counter()
+        A = spk.powerlaw_spd(args.powerlaw_n, 42)
+        nb, nl = A.plan_info()
+        log(f"(b) powerlaw_spd({args.powerlaw_n}): n={A.nrows} nnz={A.nnz} "
+            f"({A.nnz / A.nrows:.1f}/row, {nl} long rows) built in "
+            f"{time.perf_counter() - t0:.2f} s")
+        inputs.append(Input(spk, f"power-law SPD n={args.powerlaw_n}", A))
+
     timer = spk.Timer()
-
-    def step():
-        call_zero = spk._lib.call
-        call_zero("spk_memset", x.ptr, 0, x.nbytes)
-        return spk.conjugate_gradient(b, x, cfg, A, J, spk.identity)
-
     for _ in range(args.warmup):
-        s = step()
+        for inp in inputs:
+            inp.solve(spk)
     spk.synchronize()
-    times, iters = [], []
+    step_ms, per = [], {inp.name: {"ms": [], "it": []} for inp in inputs}
     t_all0 = time.perf_counter()
     for k in range(args.steps):
         timer.start()
-        s = step()
+        for inp in inputs:
+            t0 = time.perf_counter()
+            s = inp.solve(spk)
+            per[inp.name]["ms"].append((time.perf_counter() - t0) * 1e3)
+            per[inp.name]["it"].append(s.computed_iterations)
+            assert s.converged, f"{inp.name} did not converge"
         ms = timer.stop()
-        times.append(ms)
-        iters.append(s.computed_iterations)
-        log(f"step {k}: {ms:.2f} ms, {s.computed_iterations} iterations, "
-            f"converged={s.converged}")
+        step_ms.append(ms)
+        log(f"step {k}: {ms:.2f} ms " + ", ".join(
+            f"{nm}: {v['it'][-1]} it {v['ms'][-1]:.1f} ms"
+            for nm, v in per.items()))
     spk.synchronize()
     wall = time.perf_counter() - t_all0
-    relres = spk.true_relres(A, b, x)
-    ms_step = float(np.mean(times))
-    bpcg = b_spmv(n, nnz) + 11 * 8 * n
-    value = np.mean(iters) * bpcg / (ms_step * 1e-3) / 1e9
+    ms_step = float(np.mean(step_ms))
+    total_bytes = sum(np.mean(per[inp.name]["it"]) * b_pcg(inp.n, inp.nnz)
+                      for inp in inputs)
+    value = total_bytes / (ms_step * 1e-3) / 1e9
 
-    # SpMV kernel alone: back-to-back launches on the library stream
-    y = spk.DeviceVector(n)
-    for _ in range(5):
-        A.spmv(x, y)
-    timer.start()
-    for _ in range(args.spmv_reps):
-        A.spmv(x, y)
-    spmv_ms = timer.stop() / args.spmv_reps
-    bs = b_spmv(n, nnz)
-    spmv_gbs = bs / (spmv_ms * 1e-3) / 1e9
-    log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s "
-        f"({spmv_gbs / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+    results, dom, dom_share = [], None, -1.0
+    for inp in inputs:
+        us = spmv_time(spk, inp.A, inp.x, args.spmv_reps) * 1e3
+        bs = b_spmv(inp.n, inp.nnz)
+        gbs = bs / (us * 1e-6) / 1e9
+        relres = spk.true_relres(inp.A, inp.b, inp.x)
+        it = int(np.median(per[inp.name]["it"]))
+        share = it * us * 1e-3 / ms_step
+        r = {"input": inp.name, "rows": inp.n, "nnz": inp.nnz,
+             "pcg_iterations": it,
+             "pcg_time_to_1e-8_ms": round(float(np.mean(per[inp.name]["ms"])), 3),
+             "true_relres": relres,
+             "spmv_us": round(us, 2), "spmv_GBs": round(gbs, 1),
+             "spmv_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+             "spmv_bytes": bs}
+        results.append(r)
+        log(f"{inp.name}: spmv {us:.1f} us {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.1%}), relres {relres:.2e}")
+        if share > dom_share:
+            dom, dom_share = r, share
 
-    # device-to-device copy of the same size as one SpMV's traffic: the
-    # practical streaming ceiling next to the 8 TB/s peak
-    cbytes = (bs // 2) // 8 * 8
+    # device-to-device copy (the runtime's blit kernel): practical ceiling
+    cbytes = 256 * 1024 * 1024
     ca, cb = spk.DeviceVector(cbytes // 8), spk.DeviceVector(cbytes // 8)
     spk._lib.call("spk_memset", ca.ptr, 0, cbytes)
     for _ in range(3):
@@ -136,13 +190,11 @@ def main():
     timer.start()
     for _ in range(20):
         spk._lib.call("spk_memcpy_dtod", cb.ptr, ca.ptr, cbytes)
-    copy_ms = timer.stop() / 20
-    copy_gbs = 2 * cbytes / (copy_ms * 1e-3) / 1e9
+    copy_gbs = 2 * cbytes / (timer.stop() / 20 * 1e-3) / 1e9
     log(f"copy: {copy_gbs:.0f} GB/s (read+write)")
 
     out = {
-        "metric": "time-to-1e-8 residual: learned-FGMRES vs Jacobi-PCG; "
-                  "SpMV GB/s % HBM peak",
+        "metric": METRIC,
         "value": round(float(value), 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -153,29 +205,22 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (device-generated 7-point Poisson, splitmix64 RHS)",
-        "config": {"workload": f"FP64 CSR SpMV + Jacobi-PCG to 1e-8, 3D "
-                               f"7-point Poisson {g}^3",
-                   "rows": n, "nnz": nnz, "index": "int32",
-                   "pcg_iterations": int(np.median(iters)),
-                   "pcg_time_to_1e-8_ms": round(ms_step, 3),
-                   "true_relres": relres,
+        "data": "synthetic (device-generated matrices, splitmix64 RHS)",
+        "config": {"workload": "FP64 CSR SpMV + Jacobi-PCG to 1e-8 on " +
+                               " and ".join(i.name for i in inputs),
+                   "index": "int32", "inputs": results,
                    "parallelism": "1 GPU"},
-        "spmv": {"us_per_launch": round(spmv_ms * 1e3, 2),
-                 "GBs": round(spmv_gbs, 1),
-                 "frac_hbm_peak": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                 "bytes_per_launch": bs,
-                 "frac_of_copy": round(spmv_gbs / copy_gbs, 4)},
-        "copy_GBs": round(copy_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": f"spmv_kernel ({dom['input']})",
+                     "achieved": dom["spmv_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s",
+                     "frac": round(dom["spmv_GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": None},
+        "copy_GBs": round(copy_gbs, 1),
         "wall_s_timed": round(wall, 3),
     }
-    if not args.no_cpu:
+    if not args.no_cpu and "a" in args.inputs:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(g, args.cpu_iters)
+        out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)
     print(json.dumps(out), flush=True)
 
 

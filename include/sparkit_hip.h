@@ -207,6 +207,40 @@ spk_status spk_bicgstab_solve(spk_csr A, spk_precond M_left,
                               void* d_x, spk_summary* summary,
                               double* history);
 
+/* Mixed-precision iterative refinement (no reference counterpart; the
+ * closest structure is the outer correction loop of richardson.hpp:112):
+ *   r = b - A x in FP64; stop when ||r|| / ||b|| < tolerance (FP64 true
+ *   residual);  solve A d = r / ||r|| in FP32 with the inner Krylov method
+ *   (FP32 vectors and SpMV, FP64 dot products, Arnoldi/Givens scalars) to
+ *   inner_tolerance;  x += ||r|| d in FP64.
+ * inner_method: SPK_IR_GMRES (M as right preconditioner), SPK_IR_FGMRES,
+ * SPK_IR_CG (M as left preconditioner), SPK_IR_BICGSTAB (M right). */
+enum { SPK_IR_GMRES = 0, SPK_IR_FGMRES = 1, SPK_IR_CG = 2, SPK_IR_BICGSTAB = 3 };
+
+typedef struct {
+  double tolerance;       /* FP64 true relative residual target */
+  int64_t max_outer;      /* refinement steps */
+  double inner_tolerance; /* relative tolerance of each FP32 inner solve */
+  int64_t restart;        /* inner restart dimension / restart_iterations */
+  int64_t max_inner;      /* inner iteration budget per refinement step */
+  int32_t inner_method;
+  int32_t pad;
+} spk_ir_config;
+
+typedef struct {
+  double relres;                 /* final FP64 ||b - A x|| / ||b|| */
+  int64_t outer_iterations;      /* refinement steps taken */
+  int64_t inner_iterations;      /* total inner iterations */
+  int32_t converged;
+  int32_t status;
+} spk_ir_summary;
+
+/* history (host, max_outer + 1 doubles, may be NULL): the FP64 true
+ * relative residual before each refinement step and at the end. */
+spk_status spk_ir_solve(spk_csr A, spk_precond M, const spk_ir_config* cfg,
+                        const double* d_b, double* d_x, spk_ir_summary* out,
+                        double* history);
+
 /* ||b - A x||_2 / ||b||_2 in FP64 (the true-residual check). */
 spk_status spk_true_relres(spk_csr A, const double* d_b, const double* d_x,
                            double* relres);
@@ -228,6 +262,12 @@ spk_status spk_gen_poisson_3d(int64_t nx, int64_t ny, int64_t nz,
  * (matgen.hpp:274-325) to z with a Dirichlet diagonal 6d + cx + cy + cz. */
 spk_status spk_gen_convdiff_3d(int64_t nx, int64_t ny, int64_t nz, double d,
                                double cx, double cy, double cz, spk_csr* out);
+/* Power-law SPD matrix: row lengths from a Zipf(1.5) law on [3, 4096],
+ * uniform columns, values uniform [-1, 0), symmetric part (A + A^T)/2 on the
+ * union pattern, diagonal = sum |off-diagonal| + 1 (random_sparse's
+ * dominance rule, matgen.hpp:657-658).  Built on the device (sort, unique,
+ * scan); no reference counterpart. */
+spk_status spk_gen_powerlaw_spd(int64_t n, uint64_t seed, spk_csr* out);
 
 #ifdef __cplusplus
 }

@@ -143,6 +143,12 @@ def _declare(L):
     L.orc_gen_convdiff_3d_csr.argtypes = [C.c_int64] * 3 + [C.c_double] * 4 + \
         [C.c_void_p] * 3
     L.orc_gen_convdiff_3d_csr.restype = C.c_int64
+    L.orc_powerlaw_cdf.argtypes = [_f64p]
+    L.orc_powerlaw_length.argtypes = [_f64p, C.c_uint64, C.c_int64]
+    L.orc_powerlaw_length.restype = C.c_int64
+    L.orc_gen_powerlaw_spd.argtypes = [C.c_int64, C.c_uint64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
+    L.orc_gen_powerlaw_spd.restype = C.c_int64
 
 
 class OracleError(RuntimeError):
@@ -483,6 +489,30 @@ def convdiff_3d(nx, ny, nz, d, cx, cy, cz):
     counterpart; extends matgen.hpp:292-321 to 3D)."""
     return _csr_gen("orc_gen_convdiff_3d_csr", nx * ny * nz, nx, ny, nz, d,
                     cx, cy, cz)
+
+
+def powerlaw_cdf():
+    cdf = np.zeros(4096 - 3 + 1, np.float64)
+    lib().orc_powerlaw_cdf(cdf)
+    return cdf
+
+
+def powerlaw_lengths(n, seed):
+    cdf = powerlaw_cdf()
+    return np.array([lib().orc_powerlaw_length(cdf, seed, i)
+                     for i in range(n)], np.int64)
+
+
+def powerlaw_spd(n, seed):
+    """Power-law SPD matrix (no reference counterpart; definition in
+    oracle_gen.c)."""
+    L = lib()
+    nnz = L.orc_gen_powerlaw_spd(n, seed, None, None, None)
+    rp = np.zeros(n + 1, np.int64)
+    ci = np.zeros(nnz, np.int64)
+    v = np.zeros(nnz, np.float64)
+    L.orc_gen_powerlaw_spd(n, seed, _ptr(rp), _ptr(ci), _ptr(v))
+    return Csr(n, n, rp, ci, v)
 
 
 def splitmix64_at(seed, k):

@@ -290,6 +290,15 @@ int64_t orc_gen_convdiff_3d_csr(int64_t nx, int64_t ny, int64_t nz, double d,
                                 int64_t* row_ptr, int64_t* col_ind,
                                 double* vals);
 
+/* Power-law SPD matrix (definition in oracle_gen.c's header comment).
+ * orc_powerlaw_cdf fills the 4094-entry Zipf(1.5) table for lengths 3..4096;
+ * orc_powerlaw_length draws row i's target length; orc_gen_powerlaw_spd
+ * writes the CSR arrays (pass NULL arrays to get nnz). */
+void orc_powerlaw_cdf(double* cdf);
+int64_t orc_powerlaw_length(const double* cdf, uint64_t seed, int64_t i);
+int64_t orc_gen_powerlaw_spd(int64_t n, uint64_t seed, int64_t* row_ptr,
+                             int64_t* col_ind, double* vals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,9 @@ __all__ = [
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "true_relres",
     "F32", "F64", "SpkError", "InvalidArgument", "LogicError", "Timer",
-    "poisson_2d", "poisson_3d", "convdiff_3d", "uniform_pm1",
+    "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
+    "IR_BICGSTAB",
+    "poisson_2d", "poisson_3d", "convdiff_3d", "powerlaw_spd", "uniform_pm1",
 ]
 
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
@@ -399,6 +401,46 @@ def bicgstab(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
                                        C.byref(c), b.ptr, x.ptr), cfg)
 
 
+IR_GMRES, IR_FGMRES, IR_CG, IR_BICGSTAB = 0, 1, 2, 3
+
+
+@dataclass
+class IrConfig:
+    """Mixed-precision iterative refinement: FP64 true residual outside,
+    FP32 Krylov inside (FP64 dots/scalars)."""
+    tolerance: float = 1e-10
+    max_outer: int = 20
+    inner_tolerance: float = 1e-4
+    restart: int = 50
+    max_inner: int = 500
+    inner_method: int = IR_GMRES
+
+
+@dataclass
+class IrSummary:
+    relres: float
+    outer_iterations: int
+    inner_iterations: int
+    converged: bool
+    relres_history: list
+
+
+def ir_solve(b: DeviceVector, x: DeviceVector, cfg: IrConfig, A: CsrMatrix,
+             M=identity) -> IrSummary:
+    """x (FP64, updated in place) from b (FP64) by iterative refinement."""
+    if b.dtype != np.float64 or x.dtype != np.float64:
+        raise InvalidArgument(1, "ir_solve", "b and x must be float64")
+    c = _lib.IrConfig(float(cfg.tolerance), int(cfg.max_outer),
+                      float(cfg.inner_tolerance), int(cfg.restart),
+                      int(cfg.max_inner), int(cfg.inner_method), 0)
+    s = _lib.IrSummary()
+    hist = np.zeros(cfg.max_outer + 2, np.float64)
+    call("spk_ir_solve", A.h, _h(M), C.byref(c), b.ptr, x.ptr, C.byref(s),
+         hist.ctypes.data_as(C.c_void_p))
+    return IrSummary(s.relres, s.outer_iterations, s.inner_iterations,
+                     bool(s.converged), hist[:s.outer_iterations + 1].tolist())
+
+
 def true_relres(A: CsrMatrix, b: DeviceVector, x: DeviceVector) -> float:
     """||b - A x|| / ||b|| in FP64 on the device."""
     r = C.c_double()
@@ -431,6 +473,12 @@ def convdiff_3d(nx, ny, nz, d, cx, cy, cz) -> CsrMatrix:
     """7-point upwind convection-diffusion, Dirichlet diagonal."""
     return _gen("spk_gen_convdiff_3d", nx, ny, nz, float(d), float(cx),
                 float(cy), float(cz))
+
+
+def powerlaw_spd(n: int, seed: int) -> CsrMatrix:
+    """Power-law SPD matrix (Zipf(1.5) row lengths, symmetrised,
+    diagonally dominant) built on the device."""
+    return _gen("spk_gen_powerlaw_spd", n, seed)
 
 
 def uniform_pm1(seed: int, n: int) -> DeviceVector:

@@ -49,6 +49,19 @@ class Summary(C.Structure):
                 ("history_len", C.c_int64)]
 
 
+class IrConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_outer", C.c_int64),
+                ("inner_tolerance", C.c_double), ("restart", C.c_int64),
+                ("max_inner", C.c_int64), ("inner_method", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class IrSummary(C.Structure):
+    _fields_ = [("relres", C.c_double), ("outer_iterations", C.c_int64),
+                ("inner_iterations", C.c_int64), ("converged", C.c_int32),
+                ("status", C.c_int32)]
+
+
 vp = C.c_void_p
 i64 = C.c_int64
 u64 = C.c_uint64
@@ -108,11 +121,14 @@ _SIGS = {
     "spk_bicgstab_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                             C.POINTER(Summary), vp], None),
     "spk_true_relres": ([vp, vp, vp, P_dbl], None),
+    "spk_ir_solve": ([vp, vp, C.POINTER(IrConfig), vp, vp,
+                      C.POINTER(IrSummary), vp], None),
     "spk_fill_uniform_pm1": ([u64, i64, vp], None),
     "spk_gen_poisson_2d": ([i64, i64, dbl, C.POINTER(vp)], None),
     "spk_gen_poisson_3d": ([i64, i64, i64, C.POINTER(vp)], None),
     "spk_gen_convdiff_3d": ([i64, i64, i64, dbl, dbl, dbl, dbl,
                              C.POINTER(vp)], None),
+    "spk_gen_powerlaw_spd": ([i64, u64, C.POINTER(vp)], None),
 }
 
 _lib = None

@@ -216,5 +216,6 @@ extern "C" spk_status spk_bicgstab_solve(spk_csr A, spk_precond M_left,
     else
       bicgstab_solve<double>(A, M_left, M_right, cfg, (const double*)d_b,
                              (double*)d_x, summary, history);
+    check_preconds(M_left, M_right);
   });
 }

@@ -228,5 +228,6 @@ extern "C" spk_status spk_cg_solve(spk_csr A, spk_precond M_left,
     else
       cg_solve<double>(A, M_left, M_right, cfg, (const double*)d_b,
                        (double*)d_x, summary, history);
+    check_preconds(M_left, M_right);
   });
 }

@@ -285,6 +285,7 @@ spk_status spk_gmres_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
     else
       gmres_solve<double>(false, A, M_left, M_right, cfg, (const double*)d_b,
                           (double*)d_x, summary, history);
+    check_preconds(M_left, M_right);
   });
 }
 
@@ -301,6 +302,7 @@ spk_status spk_fgmres_solve(spk_csr A, spk_precond M,
     else
       gmres_solve<double>(true, A, M, nullptr, cfg, (const double*)d_b,
                           (double*)d_x, summary, history);
+    check_preconds(M, nullptr);
   });
 }
 

@@ -87,36 +87,123 @@ __global__ void ilu_split_kernel(int64_t n, const int32_t* __restrict__ rp,
   lv[lk] = 1.0;
 }
 
-// forward_solve for the rows of one level (triangular_solve.hpp:42-53)
+// Triangular sweeps in ONE launch ("sync-free").  Rows are laid out in
+// dependency-level order, one thread per row, and every level is padded to a
+// multiple of the wavefront size, so the rows of one wavefront never depend
+// on each other: a lane only ever waits for rows of EARLIER wavefronts.
+// (Without the padding a wavefront straddling two levels deadlocks: the
+// compiler may sink the final store below the wait loop, where it runs only
+// after every lane of the wavefront has left the loop.)  Every value of the
+// output doubles as its own ready flag: the output starts filled with a NaN
+// sentinel that arithmetic never produces, a finished row stores its value
+// with one agent-scope atomic store, and a row waiting on a dependency spins
+// on an agent-scope atomic load of it.  Workgroups are dispatched in order,
+// so the earliest unfinished wavefront always has its dependencies resident
+// or finished.  A bounded spin sets an error flag and ends the row, so every
+// wavefront exits whatever happens.
 template <typename T>
-__global__ void lower_level_kernel(const int32_t* __restrict__ rows, int count,
-                                   const int32_t* __restrict__ rp,
-                                   const int32_t* __restrict__ ci,
-                                   const T* __restrict__ v, const T* b, T* x) {
-#pragma clang fp contract(off)
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int i = rows[t];
-  const int pe = rp[i + 1] - 1;
-  T s = b[i];
-  for (int p = rp[i]; p < pe; ++p) s = s - v[p] * x[ci[p]];
-  x[i] = s / v[pe];
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+  typedef unsigned long long U;
+  static constexpr U bits = 0xFFF8DEADBEEFCAFEull;
+};
+template <>
+struct Sentinel<float> {
+  typedef unsigned int U;
+  static constexpr U bits = 0xFFC0DEADu;
+};
+
+template <typename T>
+__device__ __forceinline__ bool load_ready(const T* p, T* out) {
+  typedef typename Sentinel<T>::U U;
+  U u = __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(T));
+  *out = v;
+  return u != Sentinel<T>::bits;
 }
 
-// backward_solve for the rows of one level (triangular_solve.hpp:82-96)
 template <typename T>
-__global__ void upper_level_kernel(const int32_t* __restrict__ rows, int count,
-                                   const int32_t* __restrict__ rp,
-                                   const int32_t* __restrict__ ci,
-                                   const T* __restrict__ v, const T* b, T* x) {
+__device__ __forceinline__ void store_ready(T* p, T v) {
+  typedef typename Sentinel<T>::U U;
+  U u;
+  __builtin_memcpy(&u, &v, sizeof(T));
+  __hip_atomic_store(reinterpret_cast<U*>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ T sentinel_value() {
+  typedef typename Sentinel<T>::U U;
+  U u = Sentinel<T>::bits;
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(T));
+  return v;
+}
+
+constexpr unsigned kSpinLimit = 1u << 22;
+
+// LOWER: forward_solve (triangular_solve.hpp:42-53), the diagonal is the
+// row's last entry; otherwise backward_solve (:82-96), the diagonal is the
+// first.  x[c] -= v * x[c'] in storage order, multiply-adds not contracted:
+// bit-identical to the sequential reference.  Optionally resets b[i] to the
+// sentinel after reading it (b is the previous sweep's output) and marks
+// mark[i] with the sentinel (the next sweep's output).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void sweep_kernel(
+    const int32_t* __restrict__ rows, int n, const int32_t* __restrict__ rp,
+    const int32_t* __restrict__ ci, const T* __restrict__ v, T* b, T* x,
+    bool reset_b, T* mark, int* error) {
 #pragma clang fp contract(off)
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
   const int i = rows[t];
-  const int p0 = rp[i];
+  if (i < 0) return;  // level padding
+  const int p0 = rp[i], p1 = rp[i + 1];
+  const int q1 = LOWER ? p1 - 1 : p1;
+  const int pd = LOWER ? p1 - 1 : p0;
   T s = b[i];
-  for (int p = p0 + 1; p < rp[i + 1]; ++p) s = s - v[p] * x[ci[p]];
-  x[i] = s / v[p0];
+  if (reset_b) b[i] = sentinel_value<T>();
+  if (mark) mark[i] = sentinel_value<T>();
+  int p = LOWER ? p0 : p0 + 1;
+  unsigned spins = 0;
+  bool done = false;
+  while (!done) {
+    if (p < q1) {
+      T xv;
+      if (load_ready(x + ci[p], &xv)) {
+        s = s - v[p] * xv;
+        ++p;
+      } else if (++spins > kSpinLimit) {
+        atomicOr(error, 1);
+        p = q1;
+      }
+    }
+    if (p >= q1) {
+      store_ready(x + i, s / v[pd]);
+      done = true;
+    }
+  }
+}
+
+template <typename T>
+__global__ void fill_sentinel_kernel(T* x, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = sentinel_value<T>();
+}
+
+// rows grouped by level, each level padded with -1 to a multiple of 64
+std::vector<int32_t> pad_levels(const std::vector<int32_t>& rows,
+                                const std::vector<int32_t>& ptr) {
+  std::vector<int32_t> out;
+  out.reserve(rows.size() + 64 * (ptr.size() - 1));
+  for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+    for (int32_t k = ptr[l]; k < ptr[l + 1]; ++k) out.push_back(rows[k]);
+    while (out.size() % 64) out.push_back(-1);
+  }
+  return out;
 }
 
 // group rows by level (counting sort); returns level pointer array
@@ -172,27 +259,45 @@ const float* spk_precond_s::inv_diag32() {
   return inv_d32.p;
 }
 
+template <typename T>
+static T* sentinel_buffer(DevBuf<T>& buf, int64_t n) {
+  if (buf.p == nullptr || buf.n < (size_t)n) {
+    buf.alloc((size_t)n);
+    hipLaunchKernelGGL((fill_sentinel_kernel<T>), dim3(blocks_for(n)),
+                       dim3(256), 0, stream(), buf.p, n);
+    SPK_HIP_TRY(hipGetLastError());
+  }
+  return buf.p;
+}
+
 template <>
 double* spk_precond_s::tmp<double>() {
-  tmp64.reserve((size_t)n);
-  return tmp64.p;
+  return sentinel_buffer(tmp64, n);
 }
 template <>
 float* spk_precond_s::tmp<float>() {
-  tmp32.reserve((size_t)n);
-  return tmp32.p;
+  return sentinel_buffer(tmp32, n);
+}
+
+void spk_precond_s::check_sweeps() {
+  if (kind != kIlu) return;
+  int h = 0;
+  SPK_HIP_TRY(hipMemcpyAsync(&h, sweep_error.p, sizeof(int),
+                             hipMemcpyDeviceToHost, stream()));
+  SPK_HIP_TRY(hipStreamSynchronize(stream()));
+  SPK_REQUIRE(!h, SPK_HIP_ERROR,
+              "ILU triangular sweep: bounded wait expired (results invalid)");
 }
 
 template <typename T>
 void spk_precond_s::forward(const T* b, T* x) {
   hipStream_t s = stream();
   CsrView<T> v = L->view<T>();
-  for (size_t l = 0; l + 1 < lvl_ptr_l.size(); ++l) {
-    int cnt = lvl_ptr_l[l + 1] - lvl_ptr_l[l];
-    hipLaunchKernelGGL((lower_level_kernel<T>), dim3(blocks_for(cnt)),
-                       dim3(256), 0, s, lvl_rows_l.p + lvl_ptr_l[l], cnt, v.rp,
-                       v.ci, v.v, b, x);
-  }
+  hipLaunchKernelGGL((fill_sentinel_kernel<T>), dim3(blocks_for(n)),
+                     dim3(256), 0, s, x, n);
+  hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(blocks_for(sweep_len_l)),
+                     dim3(256), 0, s, sweep_rows_l.p, (int)sweep_len_l, v.rp,
+                     v.ci, v.v, (T*)b, x, false, (T*)nullptr, sweep_error.p);
   SPK_HIP_TRY(hipGetLastError());
 }
 
@@ -200,12 +305,11 @@ template <typename T>
 void spk_precond_s::backward(const T* b, T* x) {
   hipStream_t s = stream();
   CsrView<T> v = U->view<T>();
-  for (size_t l = 0; l + 1 < lvl_ptr_u.size(); ++l) {
-    int cnt = lvl_ptr_u[l + 1] - lvl_ptr_u[l];
-    hipLaunchKernelGGL((upper_level_kernel<T>), dim3(blocks_for(cnt)),
-                       dim3(256), 0, s, lvl_rows_u.p + lvl_ptr_u[l], cnt, v.rp,
-                       v.ci, v.v, b, x);
-  }
+  hipLaunchKernelGGL((fill_sentinel_kernel<T>), dim3(blocks_for(n)),
+                     dim3(256), 0, s, x, n);
+  hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(blocks_for(sweep_len_u)),
+                     dim3(256), 0, s, sweep_rows_u.p, (int)sweep_len_u, v.rp,
+                     v.ci, v.v, (T*)b, x, false, (T*)nullptr, sweep_error.p);
   SPK_HIP_TRY(hipGetLastError());
 }
 
@@ -223,10 +327,22 @@ void spk_precond_s::apply(const T* in, T* out) {
       break;
     }
     case kIlu: {
-      // ilu_apply: y = forward_solve(L, r); z = backward_solve(U, y)
+      // ilu_apply: y = forward_solve(L, r); z = backward_solve(U, y).
+      // Two launches: the forward sweep marks `out` with the sentinel after
+      // reading in[i] (in may alias out), the backward sweep returns y to
+      // the sentinel as it consumes it.
       T* y = tmp<T>();
-      forward(in, y);
-      backward((const T*)y, out);
+      hipStream_t s = stream();
+      CsrView<T> lv = L->view<T>(), uv = U->view<T>();
+      hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(blocks_for(sweep_len_l)),
+                         dim3(256), 0, s, sweep_rows_l.p, (int)sweep_len_l,
+                         lv.rp, lv.ci, lv.v, (T*)in, y, false, out,
+                         sweep_error.p);
+      hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(blocks_for(sweep_len_u)),
+                         dim3(256), 0, s, sweep_rows_u.p, (int)sweep_len_u,
+                         uv.rp, uv.ci, uv.v, y, out, true, (T*)nullptr,
+                         sweep_error.p);
+      SPK_HIP_TRY(hipGetLastError());
       break;
     }
     case kSpmv:
@@ -348,11 +464,25 @@ spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out) {
     try {
       M->kind = spk_precond_s::kIlu;
       M->n = n;
+      M->sweep_error.alloc(1);
+      SPK_HIP_TRY(hipMemsetAsync(M->sweep_error.p, 0, sizeof(int), s));
       std::vector<int32_t> rows_l, rows_u;
       M->lvl_ptr_l = group_levels(lvl_l, rows_l);
       M->lvl_ptr_u = group_levels(lvl_u, rows_u);
       M->lvl_rows_l.alloc((size_t)n);
       M->lvl_rows_u.alloc((size_t)n);
+      std::vector<int32_t> pad_l = pad_levels(rows_l, M->lvl_ptr_l);
+      std::vector<int32_t> pad_u = pad_levels(rows_u, M->lvl_ptr_u);
+      SPK_REQUIRE(pad_l.size() < (size_t)INT32_MAX && pad_u.size() < (size_t)INT32_MAX,
+                  SPK_INVALID_ARGUMENT, "too many rows for the sweep order");
+      M->sweep_len_l = (int64_t)pad_l.size();
+      M->sweep_len_u = (int64_t)pad_u.size();
+      M->sweep_rows_l.alloc(pad_l.size());
+      M->sweep_rows_u.alloc(pad_u.size());
+      SPK_HIP_TRY(hipMemcpyAsync(M->sweep_rows_l.p, pad_l.data(), pad_l.size() * 4,
+                                 hipMemcpyHostToDevice, s));
+      SPK_HIP_TRY(hipMemcpyAsync(M->sweep_rows_u.p, pad_u.data(), pad_u.size() * 4,
+                                 hipMemcpyHostToDevice, s));
       SPK_HIP_TRY(hipMemcpyAsync(M->lvl_rows_l.p, rows_l.data(), n * 4,
                                  hipMemcpyHostToDevice, s));
       SPK_HIP_TRY(hipMemcpyAsync(M->lvl_rows_u.p, rows_u.data(), n * 4,
@@ -400,6 +530,7 @@ spk_status spk_precond_apply(spk_precond M, int dtype, const void* d_in,
     SPK_REQUIRE(M && d_in && d_out, SPK_INVALID_ARGUMENT, "null argument");
     if (dtype == SPK_F32) M->apply((const float*)d_in, (float*)d_out);
     else M->apply((const double*)d_in, (double*)d_out);
+    M->check_sweeps();
   });
 }
 
@@ -443,6 +574,7 @@ spk_status spk_ilu0_forward(spk_precond M, int dtype, const void* d_b,
                 SPK_INVALID_ARGUMENT, "bad argument");
     if (dtype == SPK_F32) M->forward((const float*)d_b, (float*)d_x);
     else M->forward((const double*)d_b, (double*)d_x);
+    M->check_sweeps();
   });
 }
 
@@ -453,6 +585,7 @@ spk_status spk_ilu0_backward(spk_precond M, int dtype, const void* d_b,
                 SPK_INVALID_ARGUMENT, "bad argument");
     if (dtype == SPK_F32) M->backward((const float*)d_b, (float*)d_x);
     else M->backward((const double*)d_b, (double*)d_x);
+    M->check_sweeps();
   });
 }
 

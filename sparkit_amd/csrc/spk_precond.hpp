@@ -28,8 +28,13 @@ struct spk_precond_s {
   spk_csr_s* U = nullptr;
   spk::DevBuf<int32_t> lvl_rows_l, lvl_rows_u;
   std::vector<int32_t> lvl_ptr_l, lvl_ptr_u;
-  spk::DevBuf<double> tmp64;
-  spk::DevBuf<float> tmp32;
+  // the same orders with every level padded by -1 entries to a multiple of
+  // the wavefront size (the sweeps' thread -> row map)
+  spk::DevBuf<int32_t> sweep_rows_l, sweep_rows_u;
+  int64_t sweep_len_l = 0, sweep_len_u = 0;
+  spk::DevBuf<double> tmp64;  // between the sweeps; kept filled with the
+  spk::DevBuf<float> tmp32;   // ready sentinel between applications
+  spk::DevBuf<int> sweep_error;  // set if a sweep's bounded wait expired
   // SPAI
   spk_csr_s* M = nullptr;  // not owned
 
@@ -39,12 +44,15 @@ struct spk_precond_s {
   const T* inv_diag();
   template <typename T>
   void apply(const T* in, T* out);
+  // standalone sweeps (x is filled with the sentinel first)
   template <typename T>
   void forward(const T* b, T* x);
   template <typename T>
   void backward(const T* b, T* x);
   template <typename T>
   T* tmp();
+  // throws if a triangular sweep reported an expired wait (synchronises)
+  void check_sweeps();
 };
 
 template <>

@@ -35,6 +35,12 @@ inline void apply_precond(spk_precond_s* M, int64_t n, const T* in, T* out) {
   }
 }
 
+// Reports a triangular sweep whose bounded wait expired during the solve.
+inline void check_preconds(spk_precond_s* a, spk_precond_s* b) {
+  if (a) a->check_sweeps();
+  if (b) b->check_sweeps();
+}
+
 inline void check_solver_args(spk_csr_s* A, spk_precond_s* ML,
                               spk_precond_s* MR, const spk_solver_config* cfg,
                               const void* b, void* x) {

@@ -211,3 +211,13 @@ def test_fp32_vectors(O, spk, method):
                         max_it=2000, precision=spk.F32)
     assert s.converged
     assert O.true_relres(A, b, x.astype(np.float64)) < 1e-4
+
+
+def test_powerlaw_jacobi_pcg(O, spk):
+    A = O.powerlaw_spd(20000, 11)
+    b = O.uniform_pm1(42, A.nrows)
+    s, x, sr, xr = _run(spk, O, "cg", A, b, 1e-8, left="jacobi",
+                        restart=100000, max_it=2000)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 1
+    assert O.true_relres(A, b, x) < 1e-8

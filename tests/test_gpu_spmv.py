@@ -137,3 +137,25 @@ def test_true_relres(O, spk):
                         spk.DeviceVector.from_numpy(b),
                         spk.DeviceVector.from_numpy(x))
     assert abs(r - O.true_relres(A, b, x)) <= 1e-13 * r
+
+
+@pytest.mark.parametrize("n,seed", [(3000, 1), (20000, 7)])
+def test_powerlaw_generator_bit_exact(O, spk, n, seed):
+    """Device sort/unique/scan build of the power-law SPD matrix equals the
+    oracle's sequential build bit for bit."""
+    A = O.powerlaw_spd(n, seed)
+    rp, ci, v = spk.powerlaw_spd(n, seed).download()
+    assert np.array_equal(rp, A.row_ptr)
+    assert np.array_equal(ci, A.col_ind)
+    assert np.array_equal(v, A.vals)
+
+
+def test_powerlaw_spmv(O, spk):
+    """Medium and long power-law rows (several lanes per row, block-wide
+    reduction) against the oracle."""
+    A = O.powerlaw_spd(30000, 3)
+    x = O.uniform_pm1(5, A.nrows)
+    y = spk.multiply(spk.powerlaw_spd(30000, 3), spk.DeviceVector.from_numpy(x)).numpy()
+    y_ref = O.spmv(A, x)
+    absrow = np.abs(A.to_scipy()) @ np.abs(x)
+    assert np.all(np.abs(y - y_ref) <= 1e-12 * absrow)
