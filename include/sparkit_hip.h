@@ -269,6 +269,131 @@ spk_status spk_gen_convdiff_3d(int64_t nx, int64_t ny, int64_t nz, double d,
  * scan); no reference counterpart. */
 spk_status spk_gen_powerlaw_spd(int64_t n, uint64_t seed, spk_csr* out);
 
+/* 2D anisotropic variable-coefficient diffusion on the unit square, bilinear
+ * (Q1) finite elements, n_side x n_side interior nodes, Dirichlet boundary
+ * eliminated: a 9-point SPD matrix with (3 n_side - 2)^2 nonzeros.  Per-cell
+ * conductivity lognormal(0, sigma); per-matrix anisotropy eps log-uniform in
+ * [1, 1e3] and rotation theta uniform in [0, pi) drawn from the seed
+ * (isotropic != 0 forces eps = 1, theta = 0; sigma = 0 gives unit cells).
+ * The learned-preconditioner training and test family; no reference
+ * counterpart (the reference's 2D generators are the 5-point finite
+ * difference stencils of matgen.hpp:173-386). */
+spk_status spk_gen_aniso_q1_2d(int64_t n_side, uint64_t seed, double sigma,
+                               int isotropic, spk_csr* out);
+
+/* ------------------------------------------------------ learned SPAI
+ * A small MLP (24 per-row features -> 64 -> 64 -> 9, ReLU) predicts the rows
+ * of a sparse approximate inverse M on A's own pattern; M is applied as one
+ * SpMV (spk_precond_create_spmv), e.g. inside FGMRES.  The network is
+ * trained on the device (bf16 MFMA forward/backward, fp32 accumulation,
+ * Adam on fp32 master weights) to minimise (1/n) ||I - M A||_F^2.  Rows must
+ * hold a positive diagonal and at most 4 entries on each side of it
+ * (SPK_INVALID_ARGUMENT otherwise).  No reference counterpart: the
+ * reference's approximate inverses (neumann_preconditioner.hpp,
+ * least_squares_preconditioner.hpp) are polynomial; the learned one takes
+ * the same preconditioner slot (fgmres.hpp:452-470).
+ * Parameters, canonical order: W1[24][64] b1[64] W2[64][64] b2[64]
+ * W3[64][9] b3[9] (weights [in][out]).  A new model has He-uniform hidden
+ * layers and a zero output layer, i.e. it predicts the Jacobi inverse. */
+#define SPK_SPAI_NPARAM 6345
+typedef struct spk_spai_model_s* spk_spai_model;
+typedef struct {
+  int64_t epochs;          /* passes over the training matrices */
+  double learning_rate;    /* Adam step size */
+  int64_t batch_matrices;  /* matrices per Adam step */
+  uint64_t shuffle_seed;   /* per-epoch matrix order */
+} spk_spai_train_config;
+
+spk_status spk_spai_model_create(uint64_t seed, spk_spai_model* out);
+spk_status spk_spai_model_destroy(spk_spai_model m);
+spk_status spk_spai_model_get_params(spk_spai_model m, float* params);
+/* replaces the weights and resets the optimiser state */
+spk_status spk_spai_model_set_params(spk_spai_model m, const float* params);
+/* the 24 features of every row into a device array (n x 24, float) */
+spk_status spk_spai_features(spk_csr A, float* d_features);
+/* loss (1/n)||I - M A||_F^2 and its gradient (host array of
+ * SPK_SPAI_NPARAM floats, may be NULL) for one matrix; no update */
+spk_status spk_spai_loss_grad(spk_spai_model m, spk_csr A, double* loss,
+                              float* grad);
+/* epochs x ceil(count / batch) Adam steps; loss_history (host, may be NULL)
+ * receives each step's mean loss over its batch */
+spk_status spk_spai_train(spk_spai_model m, const spk_csr* mats, int64_t count,
+                          const spk_spai_train_config* cfg,
+                          double* loss_history);
+/* M on A's pattern as a new store (the caller destroys it) */
+spk_status spk_spai_predict(spk_spai_model m, spk_csr A, spk_csr* M_out);
+
+/* ------------------------------------------------------- batched PCG
+ * Thousands of independent SPD systems, stored as the diagonal blocks of
+ * one CSR store, solved by Jacobi-PCG in one persistent launch: each
+ * workgroup takes the next system (largest first) from a device queue and
+ * runs, until ||b - A x|| / ||b|| < tolerance in FP64, refinement steps of
+ * an inner Jacobi-PCG in `precision` (SPK_F32: FP32 SpMV and vectors,
+ * FP64 dot products) to inner_tolerance on the scaled residual, with an
+ * FP64 true-residual check after each step.  SPK_F64 with max_outer = 1 and
+ * inner_tolerance = tolerance is plain FP64 Jacobi-PCG per system
+ * (conjugate_gradient.hpp:122-312 with jacobi.hpp:25-50) whose convergence
+ * is judged on the true residual.  No reference counterpart for the
+ * batching itself. */
+typedef struct spk_batch_s* spk_batch;
+typedef struct {
+  double tolerance;        /* FP64 true relative residual per system */
+  double inner_tolerance;  /* inner solve, relative to its right-hand side */
+  int64_t max_outer;
+  int64_t max_inner;       /* per refinement step */
+  int32_t precision;       /* SPK_F32 or SPK_F64 inner arithmetic */
+  int32_t pad;
+} spk_batch_config;
+typedef struct {
+  int32_t block;      /* threads per workgroup: 256, 512, 1024 */
+  int32_t wg_per_cu;  /* persistent workgroups per CU */
+  int32_t lds_rows;   /* systems up to this many rows keep p in LDS */
+  int32_t pad;
+} spk_batch_tuning;
+typedef struct {
+  int64_t total_inner_iterations, max_inner_iterations;
+  int64_t total_outer_iterations, converged;
+  double max_relres;
+} spk_batch_summary;
+
+/* count systems of 2D 5-point variable-coefficient diffusion (cell-centred
+ * finite volumes, Dirichlet, lognormal(0,1) conductivity per cell,
+ * harmonic-mean faces) on side x side grids, side uniform in
+ * [side_min, side_max], all from `seed`. */
+spk_status spk_batch_create_diffusion(int64_t count, uint64_t seed,
+                                      int64_t side_min, int64_t side_max,
+                                      spk_batch* out);
+/* a batch over a copy of a block-diagonal store; row_offsets has count + 1
+ * entries; SPK_INVALID_ARGUMENT if an entry couples two systems */
+spk_status spk_batch_create(spk_csr A, int64_t count, const int64_t* row_offsets,
+                            spk_batch* out);
+spk_status spk_batch_destroy(spk_batch B);
+/* matrix: the batch's own store (valid while the batch lives) */
+spk_status spk_batch_info(spk_batch B, int64_t* count, int64_t* total_rows,
+                          int64_t* nnz, spk_csr* matrix);
+spk_status spk_batch_row_offsets(spk_batch B, int64_t* row_offsets);
+spk_status spk_batch_set_tuning(spk_batch B, const spk_batch_tuning* t);
+spk_status spk_batch_get_tuning(spk_batch B, spk_batch_tuning* t);
+/* times every candidate launch shape on `sample` systems spread over the
+ * size range (b and x are used as scratch: x is overwritten), keeps the
+ * fastest; tried / tried_ms (may be NULL, room for 36) list the candidates */
+spk_status spk_batch_autotune(spk_batch B, const spk_batch_config* cfg,
+                              const double* d_b, double* d_x, int64_t sample,
+                              spk_batch_tuning* chosen, spk_batch_tuning* tried,
+                              float* tried_ms, int64_t* ntried);
+/* x (FP64, all systems side by side) from b; x is reset to zero first.
+ * use_graph: capture the launch into a hipGraph on first use (per b, x,
+ * configuration and tuning) and replay it afterwards.  out may be NULL
+ * (then the call does not synchronise). */
+spk_status spk_batch_pcg_solve(spk_batch B, const spk_batch_config* cfg,
+                               const double* d_b, double* d_x, int use_graph,
+                               spk_batch_summary* out);
+/* per-system results of the last solve (host arrays of count entries, any
+ * may be NULL) */
+spk_status spk_batch_results(spk_batch B, int32_t* inner_iterations,
+                             int32_t* outer_iterations, double* relres,
+                             int32_t* converged);
+
 #ifdef __cplusplus
 }
 #endif

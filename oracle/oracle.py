@@ -149,6 +149,29 @@ def _declare(L):
     L.orc_gen_powerlaw_spd.argtypes = [C.c_int64, C.c_uint64, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
     L.orc_gen_powerlaw_spd.restype = C.c_int64
+    L.orc_aniso_q1_params.argtypes = [C.c_int64, C.c_uint64, C.c_double,
+                                      C.c_int, _f64p, _f64p]
+    L.orc_gen_aniso_q1_2d.argtypes = [C.c_int64, C.c_uint64, C.c_double,
+                                      C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+    L.orc_gen_aniso_q1_2d.restype = C.c_int64
+    L.orc_batched_side.argtypes = [C.c_uint64, C.c_int64, C.c_int64,
+                                   C.c_int64]
+    L.orc_batched_side.restype = C.c_int64
+    L.orc_batched_kappa.argtypes = [C.c_uint64, C.c_int64, C.c_int64, _f64p]
+    L.orc_gen_batched_diffusion.argtypes = [C.c_uint64, C.c_int64, C.c_int64,
+                                            C.c_int64, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
+    L.orc_gen_batched_diffusion.restype = C.c_int64
+    L.orc_spai_features.argtypes = [C.POINTER(CsrD), _f32p]
+    L.orc_bf16_round.argtypes = [C.c_float]
+    L.orc_bf16_round.restype = C.c_float
+    L.orc_spai_loss_grad_bf16.argtypes = [_f32p, C.POINTER(CsrD), C.c_void_p]
+    L.orc_spai_loss_grad_bf16.restype = C.c_double
+    L.orc_spai_loss_grad_exact.argtypes = [_f64p, C.POINTER(CsrD), C.c_void_p]
+    L.orc_spai_loss_grad_exact.restype = C.c_double
+    L.orc_spai_predict_bf16.argtypes = [_f32p, C.POINTER(CsrD), _f64p]
+    L.orc_spai_predict_exact.argtypes = [_f64p, C.POINTER(CsrD), _f64p]
 
 
 class OracleError(RuntimeError):
@@ -179,6 +202,13 @@ class Csr:
         return lib().orc_csr_at(self.row_ptr, self.col_ind,
                                 np.ascontiguousarray(self.vals, np.float64),
                                 r, c)
+
+    def dense(self):
+        D = np.zeros((self.nrows, self.ncols))
+        for r in range(self.nrows):
+            for k in range(self.row_ptr[r], self.row_ptr[r + 1]):
+                D[r, self.col_ind[k]] += self.vals[k]
+        return D
 
     def to_scipy(self):
         import scipy.sparse as sp
@@ -513,6 +543,109 @@ def powerlaw_spd(n, seed):
     v = np.zeros(nnz, np.float64)
     L.orc_gen_powerlaw_spd(n, seed, _ptr(rp), _ptr(ci), _ptr(v))
     return Csr(n, n, rp, ci, v)
+
+
+def aniso_q1_params(N, seed, sigma=1.5, isotropic=False):
+    """Cell conductivities ((N+1)^2) and the 4x4 element table of
+    aniso_q1_2d."""
+    kappa = np.zeros((N + 1) * (N + 1), np.float64)
+    E = np.zeros(16, np.float64)
+    lib().orc_aniso_q1_params(N, seed, sigma, int(isotropic), kappa, E)
+    return kappa, E.reshape(4, 4)
+
+
+def aniso_q1_2d(N, seed, sigma=1.5, isotropic=False):
+    """2D anisotropic variable-coefficient diffusion, Q1 finite elements on
+    N x N interior nodes (no reference counterpart; definition in
+    oracle_gen.c)."""
+    n = N * N
+    nnz = (3 * N - 2) ** 2
+    rp = np.zeros(n + 1, np.int64)
+    ci = np.zeros(nnz, np.int64)
+    v = np.zeros(nnz, np.float64)
+    got = lib().orc_gen_aniso_q1_2d(N, seed, sigma, int(isotropic), _ptr(rp),
+                                    _ptr(ci), _ptr(v))
+    assert got == nnz
+    return Csr(n, n, rp, ci, v)
+
+
+def batched_side(seed, k, smin=32, smax=256):
+    return int(lib().orc_batched_side(seed, k, smin, smax))
+
+
+def batched_kappa(seed, k, smin=32, smax=256):
+    s = batched_side(seed, k, smin, smax)
+    kap = np.zeros(s * s, np.float64)
+    lib().orc_batched_kappa(seed, k, s, kap)
+    return kap.reshape(s, s)
+
+
+def batched_diffusion(seed, k, smin=32, smax=256):
+    """System k of the batched variable-coefficient diffusion family (no
+    reference counterpart; definition in oracle_gen.c)."""
+    s = batched_side(seed, k, smin, smax)
+    n = s * s
+    rp = np.zeros(n + 1, np.int64)
+    ci = np.zeros(5 * n, np.int64)
+    v = np.zeros(5 * n, np.float64)
+    nnz = lib().orc_gen_batched_diffusion(seed, k, smin, smax, _ptr(rp),
+                                          _ptr(ci), _ptr(v))
+    return Csr(n, n, rp, ci[:nnz].copy(), v[:nnz].copy())
+
+
+# ---------------------------------------------------------------- learned SPAI
+SPAI_NPARAM = 6345
+SPAI_SHAPES = [("W1", (24, 64)), ("b1", (64,)), ("W2", (64, 64)),
+               ("b2", (64,)), ("W3", (64, 9)), ("b3", (9,))]
+
+
+def spai_unpack(params):
+    out, k = {}, 0
+    for name, shp in SPAI_SHAPES:
+        m = int(np.prod(shp))
+        out[name] = params[k:k + m].reshape(shp)
+        k += m
+    return out
+
+
+def spai_features(A):
+    X = np.zeros((A.nrows, 24), np.float32)
+    st = lib().orc_spai_features(C.byref(A._struct()), X)
+    if st:
+        raise OracleError(st, "spai_features")
+    return X
+
+
+def spai_loss_grad(params, A, exact=False):
+    """(loss, gradient) of (1/n)||I - M A||_F^2; bf16 device arithmetic
+    unless exact (float64 throughout)."""
+    g = np.zeros(SPAI_NPARAM, np.float64)
+    if exact:
+        P = np.ascontiguousarray(params, np.float64)
+        loss = lib().orc_spai_loss_grad_exact(P, C.byref(A._struct()), _ptr(g))
+    else:
+        P = np.ascontiguousarray(params, np.float32)
+        loss = lib().orc_spai_loss_grad_bf16(P, C.byref(A._struct()), _ptr(g))
+    return loss, g
+
+
+def spai_predict(params, A, exact=False):
+    """M's values on A's pattern."""
+    m = np.zeros(A.nnz, np.float64)
+    if exact:
+        st = lib().orc_spai_predict_exact(
+            np.ascontiguousarray(params, np.float64), C.byref(A._struct()), m)
+    else:
+        st = lib().orc_spai_predict_bf16(
+            np.ascontiguousarray(params, np.float32), C.byref(A._struct()), m)
+    if st:
+        raise OracleError(st, "spai_predict")
+    return m
+
+
+def bf16_round(x):
+    return np.array([lib().orc_bf16_round(float(v)) for v in np.ravel(x)],
+                    np.float32).reshape(np.shape(x))
 
 
 def splitmix64_at(seed, k):

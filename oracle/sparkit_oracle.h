@@ -299,6 +299,41 @@ int64_t orc_powerlaw_length(const double* cdf, uint64_t seed, int64_t i);
 int64_t orc_gen_powerlaw_spd(int64_t n, uint64_t seed, int64_t* row_ptr,
                              int64_t* col_ind, double* vals);
 
+/* 2D anisotropic variable-coefficient diffusion, Q1 finite elements, N x N
+ * interior nodes (definition in oracle_gen.c).  orc_aniso_q1_params fills
+ * the (N+1)^2 cell coefficients and the 4x4 element table; iso = 1 forces
+ * eps = 1, theta = 0.  Arrays hold N^2 + 1 and (3N - 2)^2 entries. */
+void orc_aniso_q1_params(int64_t N, uint64_t seed, double sigma, int iso,
+                         double* kappa, double* E);
+int64_t orc_gen_aniso_q1_2d(int64_t N, uint64_t seed, double sigma, int iso,
+                            int64_t* row_ptr, int64_t* col_ind, double* vals);
+
+/* Batched 2D variable-coefficient diffusion, system k (definition in
+ * oracle_gen.c): side s_k, cell conductivities (s_k^2) and the s_k^2-row
+ * CSR (at most 5 s_k^2 nonzeros). */
+int64_t orc_batched_side(uint64_t seed, int64_t k, int64_t smin, int64_t smax);
+void orc_batched_kappa(uint64_t seed, int64_t k, int64_t s, double* kappa);
+int64_t orc_gen_batched_diffusion(uint64_t seed, int64_t k, int64_t smin,
+                                  int64_t smax, int64_t* row_ptr,
+                                  int64_t* col_ind, double* vals);
+
+/* ------------------------------------------------- learned SPAI (oracle_spai.c)
+ * Features, forward, ||I - MA||_F^2 / n loss and its gradient, and M's
+ * values on A's pattern.  _bf16: the device arithmetic (bf16 GEMM operands,
+ * fp32 accumulation); _exact: float64 throughout (finite-difference
+ * checks).  params: the 6345 canonical parameters; grad may be NULL.  The
+ * loss functions return NaN for a matrix the slot rules reject. */
+int orc_spai_features(const orc_csr_d* A, float* X /* n x 24 */);
+float orc_bf16_round(float x);
+double orc_spai_loss_grad_bf16(const float* params, const orc_csr_d* A,
+                               double* grad);
+double orc_spai_loss_grad_exact(const double* params, const orc_csr_d* A,
+                                double* grad);
+int orc_spai_predict_bf16(const float* params, const orc_csr_d* A,
+                          double* mvals);
+int orc_spai_predict_exact(const double* params, const orc_csr_d* A,
+                           double* mvals);
+
 #ifdef __cplusplus
 }
 #endif

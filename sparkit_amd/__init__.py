@@ -39,7 +39,10 @@ __all__ = [
     "F32", "F64", "SpkError", "InvalidArgument", "LogicError", "Timer",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB",
-    "poisson_2d", "poisson_3d", "convdiff_3d", "powerlaw_spd", "uniform_pm1",
+    "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
+    "BatchedSystems",
+    "poisson_2d", "poisson_3d", "convdiff_3d", "powerlaw_spd", "aniso_q1_2d",
+    "uniform_pm1",
 ]
 
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
@@ -138,8 +141,9 @@ class CsrMatrix:
     (narrowed from the reference's int64) and double values, plus the
     row-binned SpMV plan."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owned=True):
         self.h = handle
+        self.owned = owned
         r, c, z = C.c_int64(), C.c_int64(), C.c_int64()
         call("spk_csr_shape", self.h, C.byref(r), C.byref(c), C.byref(z))
         self.nrows, self.ncols, self.nnz = r.value, c.value, z.value
@@ -178,7 +182,8 @@ class CsrMatrix:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None) and _lib._lib is not None:
+            if (getattr(self, "h", None) and getattr(self, "owned", True)
+                    and _lib._lib is not None):
                 _lib._lib.spk_csr_destroy(self.h)
         except Exception:
             pass
@@ -290,6 +295,183 @@ def spai(M: CsrMatrix) -> Preconditioner:
     h = C.c_void_p()
     call("spk_precond_create_spmv", M.h, C.byref(h))
     return Preconditioner(h, "spai", M.nrows, owner=M)
+
+
+# ---------------------------------------------------------------- learned SPAI
+
+SPAI_NPARAM = 6345
+
+
+class SpaiModel:
+    """The learned sparse-approximate-inverse network (24 features -> 64 ->
+    64 -> 9), trained on the device with bf16 MFMA kernels.  A new model
+    predicts the Jacobi inverse (zero output layer)."""
+
+    def __init__(self, seed: int = 0):
+        self.h = C.c_void_p()
+        call("spk_spai_model_create", seed, C.byref(self.h))
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(SPAI_NPARAM, np.float32)
+        call("spk_spai_model_get_params", self.h, p.ctypes.data_as(C.c_void_p))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == SPAI_NPARAM
+        call("spk_spai_model_set_params", self.h, p.ctypes.data_as(C.c_void_p))
+
+    def loss_grad(self, A: CsrMatrix):
+        """((1/n)||I - M A||_F^2, gradient) for one matrix, no update."""
+        loss = C.c_double()
+        g = np.zeros(SPAI_NPARAM, np.float32)
+        call("spk_spai_loss_grad", self.h, A.h, C.byref(loss),
+             g.ctypes.data_as(C.c_void_p))
+        return loss.value, g
+
+    def train(self, mats, epochs=1, learning_rate=1e-3, batch_matrices=1,
+              shuffle_seed=0) -> np.ndarray:
+        """Adam over the matrices; returns each step's mean loss."""
+        hs = (C.c_void_p * len(mats))(*[m.h.value for m in mats])
+        cfg = _lib.SpaiTrainConfig(epochs, learning_rate, batch_matrices,
+                                   shuffle_seed)
+        steps = epochs * ((len(mats) + batch_matrices - 1) // batch_matrices)
+        hist = np.zeros(max(steps, 1), np.float64)
+        call("spk_spai_train", self.h, hs, len(mats), C.byref(cfg),
+             hist.ctypes.data_as(C.c_void_p))
+        return hist[:steps]
+
+    def predict(self, A: CsrMatrix) -> CsrMatrix:
+        """M on A's pattern."""
+        h = C.c_void_p()
+        call("spk_spai_predict", self.h, A.h, C.byref(h))
+        return CsrMatrix(h)
+
+    def preconditioner(self, A: CsrMatrix) -> Preconditioner:
+        """z = M r with the predicted M (one SpMV)."""
+        return spai(self.predict(A))
+
+    def __del__(self):
+        try:
+            if self.h and _lib._lib is not None:
+                _lib._lib.spk_spai_model_destroy(self.h)
+        except Exception:
+            pass
+
+
+def spai_features(A: CsrMatrix) -> np.ndarray:
+    """The 24 per-row features the network reads (n x 24 float)."""
+    X = DeviceVector(A.nrows * 24, np.float32)
+    call("spk_spai_features", A.h, X.ptr)
+    return X.numpy().reshape(A.nrows, 24)
+
+
+# ---------------------------------------------------------------- batched PCG
+
+@dataclass
+class BatchConfig:
+    """Batched Jacobi-PCG: FP64 true-residual tolerance per system, inner
+    solve precision and budget."""
+    tolerance: float = 1e-8
+    inner_tolerance: float = 1e-4
+    max_outer: int = 30
+    max_inner: int = 20000
+    precision: int = F32
+
+
+class BatchedSystems:
+    """Many independent SPD systems (diagonal blocks of one store) solved by
+    one persistent batched Jacobi-PCG launch."""
+
+    def __init__(self, handle):
+        self.h = handle
+        c, t, z = C.c_int64(), C.c_int64(), C.c_int64()
+        m = C.c_void_p()
+        call("spk_batch_info", self.h, C.byref(c), C.byref(t), C.byref(z),
+             C.byref(m))
+        self.count, self.total_rows, self.nnz = c.value, t.value, z.value
+        self.matrix = CsrMatrix(m.value, owned=False)
+
+    @classmethod
+    def diffusion(cls, count, seed, side_min=32, side_max=256):
+        h = C.c_void_p()
+        call("spk_batch_create_diffusion", count, seed, side_min, side_max,
+             C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_blocks(cls, A: CsrMatrix, row_offsets):
+        ro = np.ascontiguousarray(row_offsets, np.int64)
+        h = C.c_void_p()
+        call("spk_batch_create", A.h, len(ro) - 1,
+             ro.ctypes.data_as(C.c_void_p), C.byref(h))
+        return cls(h)
+
+    def row_offsets(self):
+        ro = np.zeros(self.count + 1, np.int64)
+        call("spk_batch_row_offsets", self.h, ro.ctypes.data_as(C.c_void_p))
+        return ro
+
+    @property
+    def tuning(self):
+        t = _lib.BatchTuning()
+        call("spk_batch_get_tuning", self.h, C.byref(t))
+        return (t.block, t.wg_per_cu, t.lds_rows)
+
+    @tuning.setter
+    def tuning(self, v):
+        t = _lib.BatchTuning(int(v[0]), int(v[1]), int(v[2]), 0)
+        call("spk_batch_set_tuning", self.h, C.byref(t))
+
+    @staticmethod
+    def _cfg(cfg):
+        return _lib.BatchConfig(float(cfg.tolerance), float(cfg.inner_tolerance),
+                                int(cfg.max_outer), int(cfg.max_inner),
+                                int(cfg.precision), 0)
+
+    def autotune(self, cfg: BatchConfig, b, x, sample=512):
+        """Times the candidate launch shapes on a sample of the systems;
+        returns (chosen, [(tuning, ms), ...])."""
+        c = self._cfg(cfg)
+        ch = _lib.BatchTuning()
+        tried = (_lib.BatchTuning * 64)()
+        ms = np.zeros(64, np.float32)
+        nt = C.c_int64()
+        call("spk_batch_autotune", self.h, C.byref(c), b.ptr, x.ptr, sample,
+             C.byref(ch), tried, ms.ctypes.data_as(C.c_void_p), C.byref(nt))
+        lst = [((tried[i].block, tried[i].wg_per_cu, tried[i].lds_rows),
+                float(ms[i])) for i in range(nt.value)]
+        return (ch.block, ch.wg_per_cu, ch.lds_rows), lst
+
+    def solve(self, cfg: BatchConfig, b, x, use_graph=False, summary=True):
+        c = self._cfg(cfg)
+        s = _lib.BatchSummary()
+        call("spk_batch_pcg_solve", self.h, C.byref(c), b.ptr, x.ptr,
+             int(use_graph), C.byref(s) if summary else None)
+        if not summary:
+            return None
+        return {"total_inner_iterations": s.total_inner_iterations,
+                "max_inner_iterations": s.max_inner_iterations,
+                "total_outer_iterations": s.total_outer_iterations,
+                "converged": s.converged, "max_relres": s.max_relres}
+
+    def results(self):
+        it = np.zeros(self.count, np.int32)
+        ou = np.zeros(self.count, np.int32)
+        rr = np.zeros(self.count, np.float64)
+        cv = np.zeros(self.count, np.int32)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        call("spk_batch_results", self.h, P(it), P(ou), P(rr), P(cv))
+        return it, ou, rr, cv
+
+    def __del__(self):
+        try:
+            if self.h and _lib._lib is not None:
+                _lib._lib.spk_batch_destroy(self.h)
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- solvers
@@ -479,6 +661,15 @@ def powerlaw_spd(n: int, seed: int) -> CsrMatrix:
     """Power-law SPD matrix (Zipf(1.5) row lengths, symmetrised,
     diagonally dominant) built on the device."""
     return _gen("spk_gen_powerlaw_spd", n, seed)
+
+
+def aniso_q1_2d(n_side: int, seed: int, sigma: float = 1.5,
+                isotropic: bool = False) -> CsrMatrix:
+    """2D anisotropic variable-coefficient diffusion, bilinear finite
+    elements on n_side x n_side interior nodes (9-point SPD), conductivity
+    lognormal(0, sigma), anisotropy log-uniform [1, 1e3], random rotation."""
+    return _gen("spk_gen_aniso_q1_2d", n_side, seed, float(sigma),
+                int(isotropic))
 
 
 def uniform_pm1(seed: int, n: int) -> DeviceVector:

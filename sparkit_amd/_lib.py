@@ -56,6 +56,29 @@ class IrConfig(C.Structure):
                 ("pad", C.c_int32)]
 
 
+class SpaiTrainConfig(C.Structure):
+    _fields_ = [("epochs", C.c_int64), ("learning_rate", C.c_double),
+                ("batch_matrices", C.c_int64), ("shuffle_seed", C.c_uint64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("inner_tolerance", C.c_double),
+                ("max_outer", C.c_int64), ("max_inner", C.c_int64),
+                ("precision", C.c_int32), ("pad", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("wg_per_cu", C.c_int32),
+                ("lds_rows", C.c_int32), ("pad", C.c_int32)]
+
+
+class BatchSummary(C.Structure):
+    _fields_ = [("total_inner_iterations", C.c_int64),
+                ("max_inner_iterations", C.c_int64),
+                ("total_outer_iterations", C.c_int64),
+                ("converged", C.c_int64), ("max_relres", C.c_double)]
+
+
 class IrSummary(C.Structure):
     _fields_ = [("relres", C.c_double), ("outer_iterations", C.c_int64),
                 ("inner_iterations", C.c_int64), ("converged", C.c_int32),
@@ -129,6 +152,27 @@ _SIGS = {
     "spk_gen_convdiff_3d": ([i64, i64, i64, dbl, dbl, dbl, dbl,
                              C.POINTER(vp)], None),
     "spk_gen_powerlaw_spd": ([i64, u64, C.POINTER(vp)], None),
+    "spk_gen_aniso_q1_2d": ([i64, u64, dbl, i32, C.POINTER(vp)], None),
+    "spk_batch_create_diffusion": ([i64, u64, i64, i64, C.POINTER(vp)], None),
+    "spk_batch_create": ([vp, i64, vp, C.POINTER(vp)], None),
+    "spk_batch_destroy": ([vp], None),
+    "spk_batch_info": ([vp, P_i64, P_i64, P_i64, C.POINTER(vp)], None),
+    "spk_batch_row_offsets": ([vp, vp], None),
+    "spk_batch_set_tuning": ([vp, C.POINTER(BatchTuning)], None),
+    "spk_batch_get_tuning": ([vp, C.POINTER(BatchTuning)], None),
+    "spk_batch_autotune": ([vp, C.POINTER(BatchConfig), vp, vp, i64,
+                            C.POINTER(BatchTuning), vp, vp, P_i64], None),
+    "spk_batch_pcg_solve": ([vp, C.POINTER(BatchConfig), vp, vp, i32,
+                             C.POINTER(BatchSummary)], None),
+    "spk_batch_results": ([vp, vp, vp, vp, vp], None),
+    "spk_spai_model_create": ([u64, C.POINTER(vp)], None),
+    "spk_spai_model_destroy": ([vp], None),
+    "spk_spai_model_get_params": ([vp, vp], None),
+    "spk_spai_model_set_params": ([vp, vp], None),
+    "spk_spai_features": ([vp, vp], None),
+    "spk_spai_loss_grad": ([vp, vp, P_dbl, vp], None),
+    "spk_spai_train": ([vp, vp, i64, C.POINTER(SpaiTrainConfig), vp], None),
+    "spk_spai_predict": ([vp, vp, C.POINTER(vp)], None),
 }
 
 _lib = None

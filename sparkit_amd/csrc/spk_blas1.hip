@@ -2,6 +2,8 @@
 // device generators (stencil matrices written straight into CSR, seeded
 // right-hand sides).
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "spk_csr.hpp"
 #include "spk_vec.hpp"
@@ -140,6 +142,107 @@ spk_csr_s* build_generated(int64_t n, LenK len_kernel, FillK fill) {
   return A;
 }
 
+// 2D anisotropic variable-coefficient diffusion, bilinear (Q1) elements on
+// the unit square with the Dirichlet boundary eliminated (N x N interior
+// nodes, 9-point stencil).  The per-cell coefficients and the 4x4 element
+// table are drawn on the host with libm (exp, log, cos, sin, sqrt):
+//   sP = seed ^ 0x414E49: eps = exp(ln(1000) u01(mix(sP, 0))),
+//                         theta = pi u01(mix(sP, 1));
+//   K0 = R(theta) diag(1, eps) R(theta)^T;
+//   E[a][b] = ((kxx X[a][b] + kyy Y[a][b]) + kxy Z[a][b]) / 6 for the corner
+//   order q = dx + 2 dy (X, Y, Z: 6 x the exact integrals of the products of
+//   the bilinear basis gradients over the unit square);
+//   sK = seed ^ 0x4B4150: kappa(cell id) = exp(sigma z), z from Box-Muller
+//   with u1 = 1 - u01(mix(sK, 2 id)), u2 = u01(mix(sK, 2 id + 1)).
+// The device assembles A(p, p') = sum over the shared cells, ascending cell
+// id, of kappa * E[q_p][q_p'] (the same order on every path, so A is
+// bitwise symmetric).
+struct Q1Table {
+  double e[16];
+};
+
+inline uint64_t host_mix(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+inline double host_u01(uint64_t z) {
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+Q1Table aniso_q1_params(int64_t N, uint64_t seed, double sigma, bool iso,
+                        std::vector<double>& kappa) {
+#pragma clang fp contract(off)
+  static const double X[4][4] = {
+      {2, -2, 1, -1}, {-2, 2, -1, 1}, {1, -1, 2, -2}, {-1, 1, -2, 2}};
+  static const double Y[4][4] = {
+      {2, 1, -2, -1}, {1, 2, -1, -2}, {-2, -1, 2, 1}, {-1, -2, 1, 2}};
+  static const double Z[4][4] = {
+      {3, 0, 0, -3}, {0, -3, 3, 0}, {0, 3, -3, 0}, {-3, 0, 0, 3}};
+  double eps = 1.0, theta = 0.0;
+  if (!iso) {
+    eps = std::exp(6.907755278982137 * host_u01(host_mix(seed ^ 0x414E49ULL, 0)));
+    theta = 3.141592653589793 * host_u01(host_mix(seed ^ 0x414E49ULL, 1));
+  }
+  const double c = std::cos(theta), s = std::sin(theta);
+  const double kxx = c * c + eps * (s * s);
+  const double kyy = s * s + eps * (c * c);
+  const double kxy = (1.0 - eps) * (c * s);
+  Q1Table t;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b)
+      t.e[4 * a + b] = ((kxx * X[a][b] + kyy * Y[a][b]) + kxy * Z[a][b]) / 6.0;
+  const int64_t nc = N + 1;
+  kappa.assign((size_t)(nc * nc), 1.0);
+  if (sigma != 0.0)
+    for (int64_t id = 0; id < nc * nc; ++id) {
+      double u1 = 1.0 - host_u01(host_mix(seed ^ 0x4B4150ULL, 2 * (uint64_t)id));
+      double u2 = host_u01(host_mix(seed ^ 0x4B4150ULL, 2 * (uint64_t)id + 1));
+      double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+      kappa[(size_t)id] = std::exp(sigma * z);
+    }
+  return t;
+}
+
+__global__ void q1_len_kernel(int64_t N, int32_t* len) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N * N) return;
+  int64_t ix = p % N + 1, iy = p / N + 1;
+  int cx = 3 - (ix == 1) - (ix == N), cy = 3 - (iy == 1) - (iy == N);
+  len[p] = cx * cy;
+}
+
+__global__ void q1_fill_kernel(int64_t N, Q1Table t,
+                               const double* __restrict__ kappa,
+                               const int32_t* __restrict__ rp,
+                               int32_t* __restrict__ ci, double* __restrict__ v) {
+#pragma clang fp contract(off)
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N * N) return;
+  const int64_t ix = p % N + 1, iy = p / N + 1, nc = N + 1;
+  int32_t k = rp[p];
+  for (int oy = -1; oy <= 1; ++oy)
+    for (int ox = -1; ox <= 1; ++ox) {
+      const int64_t jx = ix + ox, jy = iy + oy;
+      if (jx < 1 || jx > N || jy < 1 || jy > N) continue;
+      double s = 0.0;
+      for (int b = 0; b <= 1; ++b)
+        for (int a = 0; a <= 1; ++a) {
+          const int64_t cx = ix - 1 + a, cy = iy - 1 + b;
+          const int64_t dxq = jx - cx, dyq = jy - cy;
+          if (dxq < 0 || dxq > 1 || dyq < 0 || dyq > 1) continue;
+          const int qp = (int)((ix - cx) + 2 * (iy - cy));
+          const int qq = (int)(dxq + 2 * dyq);
+          s += kappa[cy * nc + cx] * t.e[4 * qp + qq];
+        }
+      ci[k] = (int32_t)((jy - 1) * N + (jx - 1));
+      v[k] = s;
+      ++k;
+    }
+}
+
 }  // namespace
 }  // namespace spk
 
@@ -273,6 +376,31 @@ spk_status spk_gen_convdiff_3d(int64_t nx, int64_t ny, int64_t nz, double d,
   double diag = ((six_d + cx) + cy) + cz;
   Stencil7 st{diag, -d - cx, -d, -d - cy, -d, -d - cz, -d};
   return gen_stencil3d(nx, ny, nz, st, out);
+}
+
+spk_status spk_gen_aniso_q1_2d(int64_t n_side, uint64_t seed, double sigma,
+                               int isotropic, spk_csr* out) {
+  return guarded([&] {
+    SPK_REQUIRE(out && n_side >= 2 && n_side < 46000 && sigma >= 0.0,
+                SPK_INVALID_ARGUMENT, "bad argument");
+    hipStream_t s = stream();
+    std::vector<double> kappa;
+    Q1Table t = aniso_q1_params(n_side, seed, sigma, isotropic != 0, kappa);
+    DevBuf<double> dk(kappa.size());
+    SPK_HIP_TRY(hipMemcpyAsync(dk.p, kappa.data(), kappa.size() * 8,
+                               hipMemcpyHostToDevice, s));
+    const int64_t N = n_side;
+    *out = build_generated(
+        N * N,
+        [&](unsigned g, int32_t* len) {
+          hipLaunchKernelGGL(q1_len_kernel, dim3(g), dim3(256), 0, s, N, len);
+        },
+        [&](unsigned g, const int32_t* rp, int32_t* ci, double* v) {
+          hipLaunchKernelGGL(q1_fill_kernel, dim3(g), dim3(256), 0, s, N, t,
+                             (const double*)dk.p, rp, ci, v);
+        });
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+  });
 }
 
 }  // extern "C"
