@@ -14,9 +14,17 @@ of each input, matrices and vectors already in HBM.
 value: algorithmic HBM GB/s sustained over the step
        (sum of iterations x B_pcg over both solves / step time;
        B_pcg = B_spmv + 11*8*n, B_spmv = nnz*12 + 4(n+1) + 16n).
-roofline: the SpMV kernel of the input whose SpMV takes the larger share of
-       the step, B_spmv per launch / its average duration measured with HIP
-       events on the library stream (the stream every kernel runs on).
+layout: before timing, each input's SpMV layout is chosen by timing the
+       plain store against 4..64 column panels (spk_csr_tune_layout; setup,
+       reported with the candidates); --no-tune keeps the plain store.
+roofline: the SpMV of the input whose SpMV takes the larger share of the
+       step: B_spmv (algorithmic bytes, independent of the layout) / the
+       time of one SpMV (all of its panel launches) measured with HIP events
+       on the library stream (the stream every kernel runs on).
+sections (after the timed steps, not part of value): poisson_2d(256) PCG,
+       mixed-precision GMRES(50)+ILU(0) on convection-diffusion 128^3,
+       learned SPAI in FGMRES(30) on anisotropic diffusion 1024^2, batched
+       PCG over 4096 systems.
 cpu_baseline: the oracle's Jacobi-PCG (CPU restatement of the reference) on
        input (a) for a bounded number of iterations, one core.
 """
@@ -69,9 +77,17 @@ def cpu_baseline(grid, iters):
 
 
 class Input:
-    def __init__(self, spk, name, A):
+    def __init__(self, spk, name, A, tune=True):
         self.name, self.A = name, A
         self.n, self.nnz = A.nrows, A.nnz
+        self.layout = None
+        if tune:  # SpMV layout (column panels) chosen by timing, setup only
+            t0 = time.perf_counter()
+            chosen, tried = A.tune_layout()
+            self.layout = {"panels": chosen,
+                           "tried_us": {str(p): round(us, 1) for p, us in tried},
+                           "tune_s": round(time.perf_counter() - t0, 2)}
+            log(f"{name}: SpMV layout {chosen} panel(s) from {tried}")
         self.b = spk.uniform_pm1(42, self.n)
         self.x = spk.DeviceVector.zeros(self.n)
         self.J = spk.jacobi(A)
@@ -95,6 +111,254 @@ def spmv_time(spk, A, x, reps):
     return timer.stop() / reps
 
 
+def _mix(seed, k):
+    """splitmix64 element k of the stream `seed` (the generators' RNG)."""
+    m = (1 << 64) - 1
+    z = (seed + (k + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return r, (time.perf_counter() - t0) * 1e3
+
+
+def mixed_precision_section(spk, grid=128, seed=42):
+    """Mixed-precision GMRES(50) + ILU(0) on 3D upwind convection-diffusion
+    (grid^3, velocity (1, 0.5, 0.25), Peclet U[10,100] from the seed) to a
+    1e-10 FP64 true residual: FP64 GMRES against FP32-inner / FP64-outer
+    iterative refinement, and the refined solution's error against the FP64
+    solve."""
+    pe = 10.0 + 90.0 * ((_mix(seed ^ 0x5045, 0) >> 11) / 2.0 ** 53)
+    h = 1.0 / (grid + 1)
+    v = np.array([1.0, 0.5, 0.25])
+    c = pe * h * v / np.linalg.norm(v)
+    A = spk.convdiff_3d(grid, grid, grid, 1.0, *c)
+    n = A.nrows
+    b = spk.uniform_pm1(seed, n)
+    F, t_ilu = _timed(lambda: spk.ilu0(A))
+    res = {"input": f"3D upwind convection-diffusion {grid}^3", "rows": n,
+           "nnz": A.nnz, "peclet": round(pe, 3), "ilu0_setup_ms": round(t_ilu, 2),
+           "target_true_relres": 1e-10}
+    for oname, ortho in (("mgs", spk.ORTHO_MGS), ("cgs2", spk.ORTHO_CGS2)):
+        x64 = spk.DeviceVector.zeros(n)
+        cfg = spk.GmresConfig(1e-10, 50, 5000, False, spk.F64, ortho)
+        spk.gmres(b, spk.DeviceVector.zeros(n),
+                  spk.GmresConfig(1e-10, 50, 50, False, spk.F64, ortho), A,
+                  spk.identity, F)  # warm-up
+        s64, t64 = _timed(lambda: spk.gmres(b, x64, cfg, A, spk.identity, F))
+        r64 = spk.true_relres(A, b, x64)
+        res[f"fp64_gmres50_ilu0_{oname}"] = {
+            "iterations": s64.computed_iterations, "converged": s64.converged,
+            "ms": round(t64, 2), "true_relres": r64}
+        xir = spk.DeviceVector.zeros(n)
+        icfg = spk.IrConfig(tolerance=1e-10, max_outer=20, inner_tolerance=1e-4,
+                            restart=50, max_inner=2000,
+                            inner_method=spk.IR_GMRES, orthogonalization=ortho)
+        spk.ir_solve(b, spk.DeviceVector.zeros(n),
+                     spk.IrConfig(1e-10, 1, 1e-2, 50, 50, spk.IR_GMRES, ortho),
+                     A, F)
+        sir, tir = _timed(lambda: spk.ir_solve(b, xir, icfg, A, F))
+        rir = spk.true_relres(A, b, xir)
+        a64, air = x64.numpy(), xir.numpy()
+        res[f"ir_fp32_gmres50_ilu0_{oname}"] = {
+            "outer_iterations": sir.outer_iterations,
+            "inner_iterations": sir.inner_iterations,
+            "converged": sir.converged, "ms": round(tir, 2), "true_relres": rir,
+            "rel_error_vs_fp64_solve": float(np.linalg.norm(air - a64) /
+                                             np.linalg.norm(a64))}
+        res[f"speedup_ir_vs_fp64_{oname}"] = round(t64 / tir, 3)
+        log(f"mixed precision ({oname}): FP64 GMRES {s64.computed_iterations}"
+            f" it {t64:.1f} ms relres {r64:.2e}; IR {sir.outer_iterations} "
+            f"outer {sir.inner_iterations} inner {tir:.1f} ms relres "
+            f"{rir:.2e}")
+    return res
+
+
+def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
+                         epochs=8, lr=2e-3, seed=4242):
+    """Learned SPAI in FGMRES(30) on 2D anisotropic variable-coefficient
+    diffusion (Q1, n_side^2 interior nodes, conductivity lognormal(0,1.5),
+    anisotropy log-uniform [1,1e3], random rotation), trained on the device
+    on train_count matrices of train_side^2 (seeds 0..train_count-1);
+    compared with Jacobi-PCG, ILU(0)-PCG and FGMRES(30) with the untrained
+    (Jacobi) network, each to a 1e-8 FP64 true residual.  FGMRES orthogonalises by
+    CGS2 (one MGS run is reported beside it)."""
+    t0 = time.perf_counter()
+    mats = [spk.aniso_q1_2d(train_side, s, 1.5) for s in range(train_count)]
+    t_gen = time.perf_counter() - t0
+    model = spk.SpaiModel(1)
+    untrained = spk.SpaiModel(1)
+    hist, t_train = _timed(lambda: model.train(mats, epochs=epochs,
+                                               learning_rate=lr,
+                                               shuffle_seed=7))
+    spk.synchronize()
+    del mats
+    A = spk.aniso_q1_2d(n_side, seed, 1.5)
+    n = A.nrows
+    b = spk.uniform_pm1(42, n)
+    Ml, t_pred = _timed(lambda: model.preconditioner(A))
+    Mj = untrained.preconditioner(A)
+    J = spk.jacobi(A)
+    res = {"input": f"2D anisotropic Q1 diffusion {n_side}^2", "rows": n,
+           "nnz": A.nnz, "train": {
+               "matrices": train_count, "side": train_side, "epochs": epochs,
+               "steps": len(hist), "learning_rate": lr,
+               "generate_s": round(t_gen, 2), "train_s": round(t_train / 1e3, 3),
+               "loss_first_epoch": float(np.mean(hist[:train_count])),
+               "loss_last_epoch": float(np.mean(hist[-train_count:]))},
+           "predict_ms": round(t_pred, 2),
+           "loss_test_learned": model.loss_grad(A)[0],
+           "loss_test_jacobi": untrained.loss_grad(A)[0]}
+
+    def run(name, fn):
+        x = spk.DeviceVector.zeros(n)
+        fn(x)  # warm-up (same solve)
+        spk._lib.call("spk_memset", x.ptr, 0, x.nbytes)
+        s, ms = _timed(lambda: fn(x))
+        rel = spk.true_relres(A, b, x)
+        it = getattr(s, "computed_iterations", None)
+        d = {"ms": round(ms, 2), "true_relres": rel}
+        if it is not None:
+            d.update(iterations=it, converged=s.converged)
+        else:
+            d.update(outer=s.outer_iterations, inner=s.inner_iterations,
+                     converged=s.converged)
+        res[name] = d
+        log(f"learned SPAI: {name}: {d}")
+
+    cg = spk.CGConfig(1e-8, 50000, 50000)
+    fg = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_CGS2)
+    fgm = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_MGS)
+    run("jacobi_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cg, A, J))
+    F, t_ilu = _timed(lambda: spk.ilu0(A))
+    res["ilu0_setup_ms"] = round(t_ilu, 2)
+    cgi = spk.CGConfig(1e-8, 5000, 5000)
+    run("ilu0_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cgi, A, F))
+    run("fgmres30_learned_fp64", lambda x: spk.fgmres(b, x, fg, A, Ml))
+    run("fgmres30_learned_fp64_mgs", lambda x: spk.fgmres(b, x, fgm, A, Ml))
+    run("fgmres30_untrained_jacobi_fp64", lambda x: spk.fgmres(b, x, fg, A, Mj))
+    icfg = spk.IrConfig(tolerance=1e-8, max_outer=30, inner_tolerance=1e-4,
+                        restart=30, max_inner=50000,
+                        inner_method=spk.IR_FGMRES,
+                        orthogonalization=spk.ORTHO_CGS2)
+    run("ir_fp32_fgmres30_learned", lambda x: spk.ir_solve(b, x, icfg, A, Ml))
+    jt = res["jacobi_pcg_fp64"]["ms"]
+    res["speedup_learned_ir_vs_jacobi_pcg"] = round(
+        jt / res["ir_fp32_fgmres30_learned"]["ms"], 3)
+    res["speedup_learned_vs_untrained_fgmres"] = round(
+        res["fgmres30_untrained_jacobi_fp64"]["ms"] /
+        res["fgmres30_learned_fp64"]["ms"], 3)
+    return res
+
+
+def poisson2d_rhs(spk, n, seed=42):
+    """splitmix64 uniform[-1,1] minus its mean: orthogonal to the constant
+    null vector of the Neumann poisson_2d (matgen.hpp:211)."""
+    b = spk.uniform_pm1(seed, n).numpy()
+    return spk.DeviceVector.from_numpy(b - b.mean())
+
+
+def poisson2d_section(spk, side=256):
+    """Jacobi-PCG on poisson_2d(side) (the reference's own CPU-runnable
+    case), to 1e-8 with restart_iterations = max_iterations = 20000."""
+    A = spk.poisson_2d(side)
+    b = poisson2d_rhs(spk, A.nrows)
+    J = spk.jacobi(A)
+    cfg = spk.CGConfig(1e-8, 20000, 20000)
+    x = spk.DeviceVector.zeros(A.nrows)
+    spk.conjugate_gradient(b, x, cfg, A, J)
+    spk._lib.call("spk_memset", x.ptr, 0, x.nbytes)
+    s, ms = _timed(lambda: spk.conjugate_gradient(b, x, cfg, A, J))
+    res = {"input": f"poisson_2d({side}) (Neumann), mean-free RHS",
+           "rows": A.nrows, "nnz": A.nnz, "iterations": s.computed_iterations,
+           "converged": s.converged, "ms": round(ms, 2),
+           "true_relres": spk.true_relres(A, b, x)}
+    log(f"poisson_2d({side}): {s.computed_iterations} it {ms:.1f} ms")
+    return res
+
+
+def cpu_poisson2d(side=256):
+    """The oracle's full Jacobi-PCG solve of the same poisson_2d case."""
+    from oracle import oracle as O
+    A = O.poisson_2d(side, side, 0.0)
+    b = O.uniform_pm1(42, A.nrows)
+    b = b - b.mean()
+    t0 = time.perf_counter()
+    _, s = O.cg(A, b, tol=1e-8, restart_iterations=20000,
+                max_iterations=20000, left="jacobi")
+    return {"config1_full_solve_s": round(time.perf_counter() - t0, 3),
+            "config1_iterations": s.computed_iterations}
+
+
+def batched_section(spk, count=4096, seed=42, smin=32, smax=256,
+                    sample=512, reps=3):
+    """Batched Jacobi-PCG over `count` 2D variable-coefficient diffusion
+    systems (sides U{smin..smax}, lognormal(0,1) conductivity, Dirichlet) in
+    one persistent hipGraph-replayed launch: FP32 SpMV/vectors inside, FP64
+    true-residual check, 1e-8 per system; launch shape from the autotuner.
+    Compared with the same kernel in FP64 throughout."""
+    (B, t_gen) = _timed(lambda: spk.BatchedSystems.diffusion(count, seed,
+                                                             smin, smax))
+    n, nnz = B.total_rows, B.nnz
+    b = spk.uniform_pm1(seed, n)
+    x = spk.DeviceVector.zeros(n)
+    cfg = spk.BatchConfig(1e-8, 1e-4, 30, 20000, spk.F32)
+    (tune, t_tune) = _timed(lambda: B.autotune(cfg, b, x, sample=sample))
+    chosen, tried = tune
+    res = {"input": f"{count} systems, 2D 5-point variable-coefficient "
+                    f"diffusion, sides U{{{smin}..{smax}}}",
+           "systems": count, "rows": n, "nnz": nnz,
+           "generate_ms": round(t_gen, 1), "autotune_ms": round(t_tune, 1),
+           "tuning": {"block": chosen[0], "wg_per_cu": chosen[1],
+                      "lds_rows": chosen[2]},
+           "tuning_candidates": len(tried)}
+    timer = spk.Timer()
+
+    def run(c, graph):
+        B.solve(c, b, x, use_graph=graph, summary=False)  # capture / warm-up
+        ms = []
+        for _ in range(reps):
+            timer.start()
+            B.solve(c, b, x, use_graph=graph, summary=False)
+            ms.append(timer.stop())
+        s = B.solve(c, b, x, use_graph=graph)
+        return float(np.median(ms)), s
+
+    ms32, s32 = run(cfg, True)
+    # bytes per inner iteration of a system with n_k rows, nnz_k entries:
+    # FP32 SpMV (nnz*(4+4) + 4(n+1)) + 11 vector words of 4 bytes
+    it, ou, rr, cv = B.results()
+    ro = B.row_offsets()
+    rows_k = np.diff(ro)
+    nnz_k = 5 * rows_k - 4 * np.sqrt(rows_k)
+    bytes_it = nnz_k * 8 + 4 * (rows_k + 1) + 44 * rows_k
+    tot = float(np.sum(it * bytes_it))
+    gbs = tot / (ms32 * 1e-3) / 1e9
+    res["fp32_inner_graph"] = {
+        "ms": round(ms32, 2), "converged": s32["converged"],
+        "max_true_relres": s32["max_relres"],
+        "inner_iterations": s32["total_inner_iterations"],
+        "max_inner_iterations": s32["max_inner_iterations"],
+        "outer_iterations": s32["total_outer_iterations"],
+        "algorithmic_GBs": round(gbs, 1),
+        "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    c64 = spk.BatchConfig(1e-8, 1e-8, 3, 20000, spk.F64)
+    ms64, s64 = run(c64, True)
+    res["fp64_graph"] = {"ms": round(ms64, 2), "converged": s64["converged"],
+                         "max_true_relres": s64["max_relres"],
+                         "inner_iterations": s64["total_inner_iterations"]}
+    res["speedup_fp32_inner_vs_fp64"] = round(ms64 / ms32, 3)
+    log(f"batched: {count} systems n={n} nnz={nnz}: tuned {chosen} "
+        f"({t_tune:.0f} ms); FP32-inner {ms32:.1f} ms "
+        f"{s32['total_inner_iterations']} it ({gbs:.0f} GB/s), FP64 "
+        f"{ms64:.1f} ms, max relres {s32['max_relres']:.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +370,18 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=30)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the plain CSR layout (no column panels)")
+    ap.add_argument("--sections", default="p2d,mp,spai,batched",
+                    help="extra measurements after the timed steps: "
+                         "p2d (poisson_2d(256) PCG), mp (mixed precision), "
+                         "spai (learned SPAI), "
+                         "batched (batched PCG); "
+                         "'none' for neither")
+    ap.add_argument("--spai-epochs", type=int, default=8)
+    ap.add_argument("--spai-train", type=int, default=512)
+    ap.add_argument("--spai-side", type=int, default=1024)
+    ap.add_argument("--mp-grid", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -123,7 +399,8 @@ def main():
         A = spk.poisson_3d(args.grid)
         log(f"(a) poisson_3d({args.grid}): n={A.nrows} nnz={A.nnz} "
             f"built in {time.perf_counter() - t0:.2f} s")
-        inputs.append(Input(spk, f"3D 7-point Poisson {args.grid}^3", A))
+        inputs.append(Input(spk, f"3D 7-point Poisson {args.grid}^3", A,
+                            not args.no_tune))
     if "b" in args.inputs:
         t0 = time.perf_counter()
         A = spk.powerlaw_spd(args.powerlaw_n, 42)
@@ -131,7 +408,8 @@ def main():
         log(f"(b) powerlaw_spd({args.powerlaw_n}): n={A.nrows} nnz={A.nnz} "
             f"({A.nnz / A.nrows:.1f}/row, {nl} long rows) built in "
             f"{time.perf_counter() - t0:.2f} s")
-        inputs.append(Input(spk, f"power-law SPD n={args.powerlaw_n}", A))
+        inputs.append(Input(spk, f"power-law SPD n={args.powerlaw_n}", A,
+                            not args.no_tune))
 
     timer = spk.Timer()
     for _ in range(args.warmup):
@@ -174,7 +452,7 @@ def main():
              "true_relres": relres,
              "spmv_us": round(us, 2), "spmv_GBs": round(gbs, 1),
              "spmv_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-             "spmv_bytes": bs}
+             "spmv_bytes": bs, "spmv_layout": inp.layout}
         results.append(r)
         log(f"{inp.name}: spmv {us:.1f} us {gbs:.0f} GB/s "
             f"({gbs / HBM_PEAK_GBS:.1%}), relres {relres:.2e}")
@@ -218,9 +496,22 @@ def main():
         "copy_GBs": round(copy_gbs, 1),
         "wall_s_timed": round(wall, 3),
     }
+    secs = args.sections.split(",")
+    if "p2d" in secs:
+        out["poisson2d_pcg"] = poisson2d_section(spk)
+    if "mp" in secs:
+        out["mixed_precision"] = mixed_precision_section(spk, args.mp_grid)
+    if "spai" in secs:
+        out["learned_spai"] = learned_spai_section(
+            spk, n_side=args.spai_side, train_count=args.spai_train,
+            epochs=args.spai_epochs)
+    if "batched" in secs:
+        out["batched_pcg"] = batched_section(spk)
     if not args.no_cpu and "a" in args.inputs:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)
+        if "p2d" in secs:
+            out["cpu_baseline"].update(cpu_poisson2d())
     print(json.dumps(out), flush=True)
 
 

@@ -246,6 +246,114 @@ class Device_spai : public Device_preconditioner {
   Device_csr M_;  // keeps the store alive
 };
 
+// The learned sparse approximate inverse: a network trained on the device
+// (bf16 MFMA kernels, Adam) to minimise ||I - M A||_F^2 that predicts M on
+// A's pattern; predict() gives a Device_spai for the preconditioner slot of
+// fgmres (fgmres.hpp:452-470).  A new network predicts the Jacobi inverse.
+class Learned_spai {
+ public:
+  static constexpr int parameter_count = SPK_SPAI_NPARAM;
+  explicit Learned_spai(uint64_t seed = 0) {
+    spk_spai_model m = nullptr;
+    check(spk_spai_model_create(seed, &m), "spk_spai_model_create");
+    h_.reset(m, [](spk_spai_model q) { spk_spai_model_destroy(q); });
+  }
+  // Adam over the matrices; returns each step's mean loss
+  std::vector<double> train(const std::vector<Device_csr>& mats, int64_t epochs,
+                            double learning_rate = 1e-3, int64_t batch = 1,
+                            uint64_t shuffle_seed = 0) {
+    std::vector<spk_csr> hs;
+    for (const auto& m : mats) hs.push_back(m.handle());
+    spk_spai_train_config c{epochs, learning_rate, batch, shuffle_seed};
+    const int64_t steps =
+        epochs * ((static_cast<int64_t>(mats.size()) + batch - 1) / batch);
+    std::vector<double> hist(static_cast<size_t>(steps > 0 ? steps : 1));
+    check(spk_spai_train(h_.get(), hs.data(), static_cast<int64_t>(hs.size()), &c,
+                         hist.data()),
+          "spk_spai_train");
+    hist.resize(static_cast<size_t>(steps));
+    return hist;
+  }
+  double loss(const Device_csr& A, std::vector<float>* grad = nullptr) const {
+    double l = 0.0;
+    if (grad) grad->resize(parameter_count);
+    check(spk_spai_loss_grad(h_.get(), A.handle(), &l, grad ? grad->data() : nullptr),
+          "spk_spai_loss_grad");
+    return l;
+  }
+  std::vector<float> parameters() const {
+    std::vector<float> p(parameter_count);
+    check(spk_spai_model_get_params(h_.get(), p.data()), "spk_spai_model_get_params");
+    return p;
+  }
+  void set_parameters(const std::vector<float>& p) {
+    if (p.size() != static_cast<size_t>(parameter_count))
+      throw std::invalid_argument("Learned_spai: wrong parameter count");
+    check(spk_spai_model_set_params(h_.get(), p.data()), "spk_spai_model_set_params");
+  }
+  Device_csr approximate_inverse(const Device_csr& A) const {
+    spk_csr M = nullptr;
+    check(spk_spai_predict(h_.get(), A.handle(), &M), "spk_spai_predict");
+    return Device_csr(M);
+  }
+  Device_spai predict(const Device_csr& A) const {
+    return Device_spai(approximate_inverse(A));
+  }
+
+ private:
+  std::shared_ptr<spk_spai_model_s> h_;
+};
+
+// Many independent SPD systems (the diagonal blocks of one store) solved by
+// batched Jacobi-PCG in one persistent launch (FP32 inner solve with an FP64
+// true-residual refinement loop, or FP64 throughout).
+class Batched_pcg {
+ public:
+  static Batched_pcg diffusion(int64_t count, uint64_t seed, int64_t side_min,
+                               int64_t side_max) {
+    spk_batch b = nullptr;
+    check(spk_batch_create_diffusion(count, seed, side_min, side_max, &b),
+          "spk_batch_create_diffusion");
+    return Batched_pcg(b);
+  }
+  Batched_pcg(const Device_csr& A, const std::vector<int64_t>& row_offsets) {
+    spk_batch b = nullptr;
+    check(spk_batch_create(A.handle(), static_cast<int64_t>(row_offsets.size()) - 1,
+                           row_offsets.data(), &b),
+          "spk_batch_create");
+    adopt(b);
+  }
+  int64_t count() const { return count_; }
+  int64_t rows() const { return rows_; }
+  spk_batch_tuning autotune(const spk_batch_config& c, const Device_vector<double>& b,
+                            Device_vector<double>& x, int64_t sample = 512) {
+    spk_batch_tuning t{};
+    check(spk_batch_autotune(h_.get(), &c, b.data(), x.data(), sample, &t, nullptr,
+                             nullptr, nullptr),
+          "spk_batch_autotune");
+    return t;
+  }
+  void set_tuning(const spk_batch_tuning& t) {
+    check(spk_batch_set_tuning(h_.get(), &t), "spk_batch_set_tuning");
+  }
+  spk_batch_summary solve(const spk_batch_config& c, const Device_vector<double>& b,
+                          Device_vector<double>& x, bool use_graph = true) {
+    spk_batch_summary s{};
+    check(spk_batch_pcg_solve(h_.get(), &c, b.data(), x.data(), use_graph ? 1 : 0, &s),
+          "spk_batch_pcg_solve");
+    return s;
+  }
+
+ private:
+  explicit Batched_pcg(spk_batch b) { adopt(b); }
+  void adopt(spk_batch b) {
+    h_.reset(b, [](spk_batch q) { spk_batch_destroy(q); });
+    check(spk_batch_info(b, &count_, &rows_, nullptr, nullptr), "spk_batch_info");
+  }
+  std::shared_ptr<spk_batch_s> h_;
+  int64_t count_ = 0, rows_ = 0;
+};
+
 struct Device_identity {
   spk_precond handle() const { return nullptr; }
   template <typename InIt, typename OutIt>
@@ -276,6 +384,8 @@ struct Gmres_config {
   size_type restart_dimension{};
   size_type max_iterations{};
   bool collect_residuals{};
+  // not in the reference: SPK_ORTHO_MGS (its algorithm) or SPK_ORTHO_CGS2
+  int orthogonalization{SPK_ORTHO_MGS};
 };
 template <typename T>
 using Fgmres_config = Gmres_config<T>;
@@ -355,6 +465,16 @@ spk_solver_config make_config(double tol, size_type restart,
   return c;
 }
 
+// orthogonalization of a config that has the field (this layer's
+// Gmres_config); the reference's own structs get MGS, its algorithm
+template <typename Config>
+int ortho_of(const Config& c) {
+  if constexpr (requires { c.orthogonalization; })
+    return static_cast<int>(c.orthogonalization);
+  else
+    return SPK_ORTHO_MGS;
+}
+
 }  // namespace detail
 
 // ------------------------------------------------------------------ solvers
@@ -395,6 +515,7 @@ Summary<T> gmres(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
                  const Right& M_right) {
   auto c = detail::make_config<T>(cfg.tolerance, cfg.restart_dimension,
                                   cfg.max_iterations, cfg.collect_residuals);
+  c.orthogonalization = detail::ortho_of(cfg);
   return detail::run_device<T>(c, [&](const spk_solver_config* pc,
                                       spk_summary* s, double* h) {
     return spk_gmres_solve(A.handle(), detail::slot(M_left),
@@ -420,6 +541,7 @@ Summary<T> fgmres(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
                   const Device_csr& A, const Prec& M) {
   auto c = detail::make_config<T>(cfg.tolerance, cfg.restart_dimension,
                                   cfg.max_iterations, cfg.collect_residuals);
+  c.orthogonalization = detail::ortho_of(cfg);
   return detail::run_device<T>(c, [&](const spk_solver_config* pc,
                                       spk_summary* s, double* h) {
     return spk_fgmres_solve(A.handle(), detail::slot(M), pc, b.data(),

@@ -107,6 +107,48 @@ spk_status spk_csr_plan_info(spk_csr A, int64_t* nblocks,
  * `dtype` (SPK_F64, or SPK_F32 using the FP32 copy of the values). */
 spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y);
 
+/* Assembly on the device from `count` (row, col[, value]) entries held in
+ * device memory (int64 indices, the reference's Index): the pattern of
+ * Compressed_row_sparsity(Shape, indices)
+ * (Compressed_row_sparsity_Impl.cpp:10-40: sorted, duplicates collapsed)
+ * and, with d_vals, make_matrix's values (matgen.hpp:25-52: for a repeated
+ * position the last entry in input order wins); d_vals NULL gives zero
+ * values.  SPK_LOGIC_ERROR for rows or cols <= 1 (Shape.cpp:8-10) or a
+ * negative index (Index.cpp:8); SPK_INVALID_ARGUMENT for an index outside
+ * the shape. */
+spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
+                            const int64_t* d_rows, const int64_t* d_cols,
+                            const double* d_vals, spk_csr* out);
+
+/* Permutations on the device, perm (device, int64, perm[old] = new) of
+ * nrows (rows, both) or ncols (columns) entries: rperm (mode
+ * SPK_PERM_ROWS), cperm (SPK_PERM_COLS: columns renamed, rows re-sorted) and
+ * dperm (SPK_PERM_BOTH, P A P^T) of permutation.hpp:41-148, as a new store.
+ * SPK_INVALID_ARGUMENT if perm is not a permutation
+ * (is_valid_permutation, permutation.cpp:15-26) or for dperm of a
+ * non-square matrix. */
+#define SPK_PERM_ROWS 1
+#define SPK_PERM_COLS 2
+#define SPK_PERM_BOTH 3
+spk_status spk_csr_permute(spk_csr A, const int64_t* d_perm, int mode,
+                           spk_csr* out);
+
+/* Layout of the store for SpMV: `panels` column panels (1: the plain
+ * store).  Each panel is a store of its own holding the entries of an equal
+ * share of the columns; an SpMV runs the panels in column order, carrying
+ * every row's running sum from one to the next, so each launch gathers from
+ * an L2-sized window of x (for matrices whose rows scatter over all columns;
+ * rows summed by one lane keep the reference's storage-order sum).  The
+ * values are copied when the panels are made: call again after changing
+ * them.  No reference counterpart (Compressed_row_matrix has one layout). */
+spk_status spk_csr_set_panels(spk_csr A, int64_t panels);
+spk_status spk_csr_get_panels(spk_csr A, int64_t* panels);
+/* times one FP64 SpMV for 1, 4, 8, ... 64 panels (those leaving at least
+ * 32768 columns per panel), keeps the fastest; tried / tried_us (may be NULL,
+ * room for 6) list the candidates */
+spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
+                               float* tried_us, int64_t* ntried);
+
 /* extract_diagonal(A) (unary.hpp:18-39): d has min(nrows, ncols) doubles. */
 spk_status spk_csr_extract_diagonal(spk_csr A, double* d_diag);
 
@@ -173,7 +215,15 @@ typedef struct {
   int64_t max_iterations;
   int32_t collect_residuals;
   int32_t precision;
+  /* (F)GMRES only: SPK_ORTHO_MGS (0, the reference's modified Gram-Schmidt,
+   * gmres.hpp:258-267) or SPK_ORTHO_CGS2 (classical Gram-Schmidt twice:
+   * the same Krylov basis up to rounding, nine launches per Arnoldi step
+   * instead of 2(j+2)+1) */
+  int32_t orthogonalization;
+  int32_t reserved;
 } spk_solver_config;
+#define SPK_ORTHO_MGS 0
+#define SPK_ORTHO_CGS2 1
 
 /* Mirrors CGSummary / Gmres_summary / ... (conjugate_gradient.hpp:26-40).
  * iteration_residuals go to the caller's `history` buffer (host, at least
@@ -224,7 +274,7 @@ typedef struct {
   int64_t restart;        /* inner restart dimension / restart_iterations */
   int64_t max_inner;      /* inner iteration budget per refinement step */
   int32_t inner_method;
-  int32_t pad;
+  int32_t orthogonalization; /* inner (F)GMRES: SPK_ORTHO_MGS / SPK_ORTHO_CGS2 */
 } spk_ir_config;
 
 typedef struct {

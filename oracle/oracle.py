@@ -155,6 +155,10 @@ def _declare(L):
                                       C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
     L.orc_gen_aniso_q1_2d.restype = C.c_int64
+    L.orc_is_valid_permutation.argtypes = [C.c_int64, _i64p]
+    L.orc_csr_permute.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p,
+                                  C.c_void_p, _i64p, C.c_int, _i64p, _i64p,
+                                  C.c_void_p]
     L.orc_batched_side.argtypes = [C.c_uint64, C.c_int64, C.c_int64,
                                    C.c_int64]
     L.orc_batched_side.restype = C.c_int64
@@ -567,6 +571,27 @@ def aniso_q1_2d(N, seed, sigma=1.5, isotropic=False):
                                     _ptr(ci), _ptr(v))
     assert got == nnz
     return Csr(n, n, rp, ci, v)
+
+
+def is_valid_permutation(perm):
+    return bool(lib().orc_is_valid_permutation(
+        len(perm), np.ascontiguousarray(perm, np.int64)))
+
+
+def permute(A, perm, mode):
+    """rperm (mode 'rows'), cperm ('cols') or dperm ('both'):
+    permutation.hpp:41-148 (perm[old] = new)."""
+    m = {"rows": 1, "cols": 2, "both": 3}[mode]
+    perm = np.ascontiguousarray(perm, np.int64)
+    rp = np.zeros(A.nrows + 1, np.int64)
+    ci = np.zeros(max(A.nnz, 1), np.int64)
+    v = np.zeros(max(A.nnz, 1), np.float64)
+    vals = np.ascontiguousarray(A.vals, np.float64)
+    st = lib().orc_csr_permute(A.nrows, A.ncols, A.row_ptr, A.col_ind,
+                               _ptr(vals), perm, m, rp, ci, _ptr(v))
+    if st:
+        raise OracleError(st, "permute")
+    return Csr(A.nrows, A.ncols, rp, ci[:A.nnz].copy(), v[:A.nnz].copy())
 
 
 def batched_side(seed, k, smin=32, smax=256):

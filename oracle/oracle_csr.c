@@ -482,3 +482,66 @@ int64_t orc_gen_convdiff_3d_csr(int64_t nx, int64_t ny, int64_t nz, double d,
 }
 
 #undef EMIT
+
+/* ------------------------------------------------------------ permutations
+ * perm maps old index -> new index.  permutation.cpp:15-35 (validity,
+ * inverse), permutation.hpp:41-148 (rperm: row r of the result is old row
+ * inv[r]; cperm: column c becomes perm[c], each row re-sorted by the new
+ * columns; dperm: both, P A P^T).  Values travel with their entries. */
+int orc_is_valid_permutation(int64_t n, const int64_t* perm) {
+  char* seen = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  int ok = 1;
+  for (int64_t k = 0; k < n && ok; ++k) {
+    if (perm[k] < 0 || perm[k] >= n || seen[perm[k]]) ok = 0;
+    else seen[perm[k]] = 1;
+  }
+  free(seen);
+  return ok;
+}
+
+typedef struct {
+  int64_t c;
+  double v;
+} orc_cv;
+
+static int cmp_cv(const void* a, const void* b) {
+  int64_t x = ((const orc_cv*)a)->c, y = ((const orc_cv*)b)->c;
+  return (x > y) - (x < y);
+}
+
+/* mode: 1 rows, 2 columns, 3 both.  Output arrays: row_ptr n+1, col_ind
+ * and vals nnz (vals may be NULL for a pattern). */
+int orc_csr_permute(int64_t nrows, int64_t ncols, const int64_t* rp,
+                    const int64_t* ci, const double* v, const int64_t* perm,
+                    int mode, int64_t* orp, int64_t* oci, double* ov) {
+  const int64_t np = (mode & 1) ? nrows : ncols;
+  if ((mode & 1) && (mode & 2) && nrows != ncols) return ORC_INVALID_ARGUMENT;
+  if (!orc_is_valid_permutation(np, perm)) return ORC_INVALID_ARGUMENT;
+  int64_t* inv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nrows > 0 ? nrows : 1));
+  for (int64_t r = 0; r < nrows; ++r) inv[r] = r;
+  if (mode & 1)
+    for (int64_t k = 0; k < nrows; ++k) inv[perm[k]] = k;
+  int64_t maxlen = 0;
+  for (int64_t r = 0; r < nrows; ++r)
+    if (rp[r + 1] - rp[r] > maxlen) maxlen = rp[r + 1] - rp[r];
+  orc_cv* row = (orc_cv*)malloc(sizeof(orc_cv) * (size_t)(maxlen > 0 ? maxlen : 1));
+  int64_t out = 0;
+  orp[0] = 0;
+  for (int64_t nr = 0; nr < nrows; ++nr) {
+    const int64_t o = inv[nr];
+    int64_t len = 0;
+    for (int64_t p = rp[o]; p < rp[o + 1]; ++p, ++len) {
+      row[len].c = (mode & 2) ? perm[ci[p]] : ci[p];
+      row[len].v = v ? v[p] : 0.0;
+    }
+    if (mode & 2) qsort(row, (size_t)len, sizeof(orc_cv), cmp_cv);
+    for (int64_t k = 0; k < len; ++k, ++out) {
+      oci[out] = row[k].c;
+      if (ov) ov[out] = row[k].v;
+    }
+    orp[nr + 1] = out;
+  }
+  free(row);
+  free(inv);
+  return ORC_OK;
+}

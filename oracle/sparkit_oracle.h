@@ -74,6 +74,15 @@ int orc_make_matrix(int64_t nrows, int64_t ncols, int64_t nent,
 double orc_csr_at(const int64_t* row_ptr, const int64_t* col_ind,
                   const double* vals, int64_t r, int64_t c);
 
+/* Permutations (perm[old] = new): permutation.cpp:15-35 and
+ * permutation.hpp:41-148.  mode 1 = rperm, 2 = cperm, 3 = dperm (P A P^T).
+ * ORC_INVALID_ARGUMENT for an invalid permutation (or dperm of a
+ * non-square matrix). */
+int orc_is_valid_permutation(int64_t n, const int64_t* perm);
+int orc_csr_permute(int64_t nrows, int64_t ncols, const int64_t* rp,
+                    const int64_t* ci, const double* v, const int64_t* perm,
+                    int mode, int64_t* orp, int64_t* oci, double* ov);
+
 /* multiply(A, x): y_i = sum_j a_ij x_cj in storage order from 0.
  * sparse_blas.hpp:17-34. */
 void orc_spmv_d(int64_t nrows, const int64_t* row_ptr, const int64_t* col_ind,

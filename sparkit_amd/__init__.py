@@ -27,8 +27,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (F32, F64, BREAKDOWN, InvalidArgument, LogicError, SpkError,
-                   call, load)
+from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS2, ORTHO_MGS, InvalidArgument,
+                   LogicError, SpkError, call, load)
 
 __all__ = [
     "init", "synchronize", "DeviceVector", "CsrMatrix", "Preconditioner",
@@ -36,7 +36,7 @@ __all__ = [
     "extract_diagonal", "jacobi_apply", "ilu_apply", "CGConfig",
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "true_relres",
-    "F32", "F64", "SpkError", "InvalidArgument", "LogicError", "Timer",
+    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
@@ -160,6 +160,21 @@ class CsrMatrix:
         return cls(h)
 
     @classmethod
+    def assemble(cls, nrows, ncols, rows, cols, vals=None):
+        """Compressed_row_sparsity(Shape, indices) / make_matrix built on
+        the device from (row, col[, value]) lists (last duplicate wins)."""
+        r = DeviceVector.from_numpy(np.ascontiguousarray(rows, np.int64)
+                                    .view(np.float64))
+        c = DeviceVector.from_numpy(np.ascontiguousarray(cols, np.int64)
+                                    .view(np.float64))
+        v = None if vals is None else DeviceVector.from_numpy(
+            np.ascontiguousarray(vals, np.float64))
+        h = C.c_void_p()
+        call("spk_csr_assemble", int(nrows), int(ncols), len(rows), r.ptr,
+             c.ptr, None if v is None else v.ptr, C.byref(h))
+        return cls(h)
+
+    @classmethod
     def from_oracle(cls, A):
         """From any object with nrows/ncols/row_ptr/col_ind/vals."""
         return cls.from_host(A.nrows, A.ncols, A.row_ptr, A.col_ind, A.vals)
@@ -176,6 +191,38 @@ class CsrMatrix:
         nb, nl = C.c_int64(), C.c_int64()
         call("spk_csr_plan_info", self.h, C.byref(nb), C.byref(nl))
         return nb.value, nl.value
+
+    def permute(self, perm, mode="both") -> "CsrMatrix":
+        """rperm ('rows'), cperm ('cols') or dperm ('both', P A P^T) of
+        permutation.hpp:41-148 (perm[old] = new), on the device."""
+        m = {"rows": 1, "cols": 2, "both": 3}[mode]
+        p = DeviceVector.from_numpy(np.ascontiguousarray(perm, np.int64)
+                                    .view(np.float64))
+        h = C.c_void_p()
+        call("spk_csr_permute", self.h, p.ptr, m, C.byref(h))
+        return CsrMatrix(h)
+
+    @property
+    def panels(self) -> int:
+        """Number of column panels the SpMV runs over (1: plain store)."""
+        p = C.c_int64()
+        call("spk_csr_get_panels", self.h, C.byref(p))
+        return p.value
+
+    def set_panels(self, panels: int):
+        call("spk_csr_set_panels", self.h, int(panels))
+
+    def tune_layout(self):
+        """Times the SpMV over 1..64 column panels and keeps the fastest;
+        returns (chosen, [(panels, us), ...])."""
+        ch, nt = C.c_int64(), C.c_int64()
+        tried = np.zeros(8, np.int64)
+        us = np.zeros(8, np.float32)
+        call("spk_csr_tune_layout", self.h, C.byref(ch),
+             tried.ctypes.data_as(C.c_void_p), us.ctypes.data_as(C.c_void_p),
+             C.byref(nt))
+        return ch.value, [(int(tried[i]), float(us[i]))
+                          for i in range(nt.value)]
 
     def spmv(self, x: DeviceVector, y: DeviceVector):
         call("spk_csr_spmv", self.h, _code(x.dtype), x.ptr, y.ptr)
@@ -489,12 +536,15 @@ class CGConfig:
 
 @dataclass
 class GmresConfig:
-    """Gmres_config (gmres.hpp:24-47)."""
+    """Gmres_config (gmres.hpp:24-47) + the arithmetic precision of the
+    Krylov vectors and the orthogonalization (ORTHO_MGS, the reference's,
+    or ORTHO_CGS2)."""
     tolerance: float = 0.0
     restart_dimension: int = 0
     max_iterations: int = 0
     collect_residuals: bool = False
     precision: int = F64
+    orthogonalization: int = 0
 
 
 FgmresConfig = GmresConfig  # Fgmres_config has the same fields
@@ -523,7 +573,8 @@ def _cfg(c, restart):
     return _lib.SolverConfig(float(c.tolerance), int(restart),
                              int(c.max_iterations),
                              1 if c.collect_residuals else 0,
-                             int(c.precision))
+                             int(c.precision),
+                             int(getattr(c, "orthogonalization", 0)), 0)
 
 
 def _run(name, args, cfg):
@@ -596,6 +647,7 @@ class IrConfig:
     restart: int = 50
     max_inner: int = 500
     inner_method: int = IR_GMRES
+    orthogonalization: int = 0
 
 
 @dataclass
@@ -614,7 +666,8 @@ def ir_solve(b: DeviceVector, x: DeviceVector, cfg: IrConfig, A: CsrMatrix,
         raise InvalidArgument(1, "ir_solve", "b and x must be float64")
     c = _lib.IrConfig(float(cfg.tolerance), int(cfg.max_outer),
                       float(cfg.inner_tolerance), int(cfg.restart),
-                      int(cfg.max_inner), int(cfg.inner_method), 0)
+                      int(cfg.max_inner), int(cfg.inner_method),
+                      int(cfg.orthogonalization))
     s = _lib.IrSummary()
     hist = np.zeros(cfg.max_outer + 2, np.float64)
     call("spk_ir_solve", A.h, _h(M), C.byref(c), b.ptr, x.ptr, C.byref(s),

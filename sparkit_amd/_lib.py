@@ -21,6 +21,9 @@ NO_DEVICE = 6
 F64 = 0
 F32 = 1
 
+ORTHO_MGS = 0
+ORTHO_CGS2 = 1
+
 
 class SpkError(RuntimeError):
     def __init__(self, status: int, where: str, msg: str):
@@ -39,7 +42,8 @@ class LogicError(SpkError):
 class SolverConfig(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("restart", C.c_int64),
                 ("max_iterations", C.c_int64),
-                ("collect_residuals", C.c_int32), ("precision", C.c_int32)]
+                ("collect_residuals", C.c_int32), ("precision", C.c_int32),
+                ("orthogonalization", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Summary(C.Structure):
@@ -53,7 +57,7 @@ class IrConfig(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_outer", C.c_int64),
                 ("inner_tolerance", C.c_double), ("restart", C.c_int64),
                 ("max_inner", C.c_int64), ("inner_method", C.c_int32),
-                ("pad", C.c_int32)]
+                ("orthogonalization", C.c_int32)]
 
 
 class SpaiTrainConfig(C.Structure):
@@ -121,6 +125,11 @@ _SIGS = {
     "spk_csr_plan_info": ([vp, P_i64, P_i64], None),
     "spk_csr_spmv": ([vp, i32, vp, vp], None),
     "spk_csr_extract_diagonal": ([vp, vp], None),
+    "spk_csr_assemble": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)], None),
+    "spk_csr_permute": ([vp, vp, i32, C.POINTER(vp)], None),
+    "spk_csr_set_panels": ([vp, i64], None),
+    "spk_csr_get_panels": ([vp, P_i64], None),
+    "spk_csr_tune_layout": ([vp, P_i64, vp, vp, P_i64], None),
     "spk_dot": ([i32, i64, vp, vp, P_dbl], None),
     "spk_axpby": ([i32, i64, dbl, vp, dbl, vp], None),
     "spk_convert": ([i64, i32, vp, i32, vp], None),

@@ -110,6 +110,101 @@ void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks) {
 
 }  // namespace spk
 
+namespace spk {
+namespace {
+
+// first position in each row whose column is >= p * width, p = 0..P
+__global__ void panel_bounds_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                    const int32_t* __restrict__ ci, int P,
+                                    int64_t width, int32_t* __restrict__ start) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r0 = rp[i], r1 = rp[i + 1];
+  for (int p = 0; p <= P; ++p) {
+    const int64_t c0 = (int64_t)p * width;
+    int lo = r0, hi = r1;
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if ((int64_t)ci[mid] < c0) lo = mid + 1; else hi = mid;
+    }
+    start[(size_t)p * n + i] = p == P ? r1 : lo;
+  }
+}
+
+__global__ void panel_len_kernel(int64_t n, const int32_t* __restrict__ start,
+                                 int p, int32_t* __restrict__ len) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  len[i] = start[(size_t)(p + 1) * n + i] - start[(size_t)p * n + i];
+}
+
+__global__ void panel_copy_kernel(int64_t n, const int32_t* __restrict__ start,
+                                  int p, const int32_t* __restrict__ ci,
+                                  const double* __restrict__ v,
+                                  const int32_t* __restrict__ prp,
+                                  int32_t* __restrict__ pci,
+                                  double* __restrict__ pv) {
+  // one wavefront per row
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int a = start[(size_t)p * n + i], b = start[(size_t)(p + 1) * n + i];
+  const int o = prp[i] - a;
+  for (int k = a + lane; k < b; k += 64) {
+    pci[k + o] = ci[k];
+    pv[k + o] = v[k];
+  }
+}
+
+}  // namespace
+
+void build_panels(spk_csr_s& A, int P) {
+  A.drop_panels();
+  if (P <= 1 || A.nrows == 0) return;
+  hipStream_t s = stream();
+  const int64_t n = A.nrows;
+  const int64_t width = (A.ncols + P - 1) / P;
+  DevBuf<int32_t> start((size_t)(P + 1) * (size_t)n);
+  DevBuf<int32_t> len((size_t)n);
+  hipLaunchKernelGGL(panel_bounds_kernel, dim3(grid_for(n)), dim3(256), 0, s, n,
+                     (const int32_t*)A.rp.p, (const int32_t*)A.ci.p, P, width,
+                     start.p);
+  SPK_HIP_TRY(hipGetLastError());
+  std::vector<int32_t> hl((size_t)n), hrp((size_t)n + 1);
+  try {
+    for (int p = 0; p < P; ++p) {
+      hipLaunchKernelGGL(panel_len_kernel, dim3(grid_for(n)), dim3(256), 0, s, n,
+                         (const int32_t*)start.p, p, len.p);
+      SPK_HIP_TRY(hipMemcpyAsync(hl.data(), len.p, n * 4, hipMemcpyDeviceToHost, s));
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      int64_t acc = 0;
+      hrp[0] = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        acc += hl[(size_t)i];
+        hrp[(size_t)i + 1] = (int32_t)acc;
+      }
+      auto* Q = new spk_csr_s();
+      A.panels.push_back(Q);
+      Q->allocate(n, A.ncols, acc);
+      SPK_HIP_TRY(hipMemcpyAsync(Q->rp.p, hrp.data(), (n + 1) * 4,
+                                 hipMemcpyHostToDevice, s));
+      if (acc > 0)
+        hipLaunchKernelGGL(panel_copy_kernel, dim3((unsigned)((n * 64 + 255) / 256)),
+                           dim3(256), 0, s, n, (const int32_t*)start.p, p,
+                           (const int32_t*)A.ci.p, (const double*)A.v64.p,
+                           (const int32_t*)Q->rp.p, Q->ci.p, Q->v64.p);
+      SPK_HIP_TRY(hipGetLastError());
+      Q->h_rp = hrp;
+      upload_plan(*Q, build_spmv_plan(Q->h_rp.data(), n, &Q->nlong));
+    }
+  } catch (...) {
+    A.drop_panels();
+    throw;
+  }
+}
+
+}  // namespace spk
+
 void spk_csr_s::allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_) {
   nrows = nrows_;
   ncols = ncols_;
@@ -280,6 +375,72 @@ spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y) {
       launch_spmv(*A, (const float*)d_x, EpiStore<float>{(float*)d_y}, s);
     else
       launch_spmv(*A, (const double*)d_x, EpiStore<double>{(double*)d_y}, s);
+  });
+}
+
+spk_status spk_csr_set_panels(spk_csr A, int64_t panels) {
+  return guarded([&] {
+    SPK_REQUIRE(A && panels >= 1 && panels <= 4096, SPK_INVALID_ARGUMENT,
+                "bad argument");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    build_panels(*A, (int)panels);
+  });
+}
+
+spk_status spk_csr_get_panels(spk_csr A, int64_t* panels) {
+  return guarded([&] {
+    SPK_REQUIRE(A && panels, SPK_INVALID_ARGUMENT, "null argument");
+    *panels = A->panels.empty() ? 1 : (int64_t)A->panels.size();
+  });
+}
+
+spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
+                               float* tried_us, int64_t* ntried) {
+  return guarded([&] {
+    SPK_REQUIRE(A && chosen, SPK_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = stream();
+    DevBuf<double> x((size_t)std::max<int64_t>(A->ncols, 1));
+    DevBuf<double> y((size_t)std::max<int64_t>(A->nrows, 1));
+    SPK_HIP_TRY(hipMemsetAsync(x.p, 0, x.n * 8, s));
+    std::vector<int> cands{1};
+    for (int P : {4, 8, 16, 32, 64})
+      if (A->ncols >= (int64_t)P * 32768) cands.push_back(P);
+    hipEvent_t e0, e1;
+    SPK_HIP_TRY(hipEventCreate(&e0));
+    SPK_HIP_TRY(hipEventCreate(&e1));
+    int best = 1;
+    float best_us = 1e30f;
+    int64_t k = 0;
+    try {
+      for (int P : cands) {
+        build_panels(*A, P);
+        for (int w = 0; w < 2; ++w) launch_spmv(*A, (const double*)x.p, EpiStore<double>{y.p}, s);
+        SPK_HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < 5; ++r)
+          launch_spmv(*A, (const double*)x.p, EpiStore<double>{y.p}, s);
+        SPK_HIP_TRY(hipEventRecord(e1, s));
+        SPK_HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SPK_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1000.f / 5.f;
+        if (tried) tried[k] = P;
+        if (tried_us) tried_us[k] = us;
+        ++k;
+        if (us < best_us) {
+          best_us = us;
+          best = P;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    build_panels(*A, best);
+    if (ntried) *ntried = k;
+    *chosen = best;
   });
 }
 

@@ -57,13 +57,42 @@ struct spk_csr_s {
   bool has32 = false;
   int nblocks = 0, nlong = 0;
   std::vector<int32_t> h_rp;  // host copy of the row offsets
+  // Column panels (empty: none).  Panel p holds the entries with columns in
+  // [p W, (p+1) W) as a store of its own; an SpMV runs the panels in column
+  // order, each continuing every row's running sum from the previous panel
+  // (pacc), so the gathers of one launch stay inside an L2-sized window of
+  // x.  Values are copied when the panels are made.
+  std::vector<spk_csr_s*> panels;
+  spk::DevBuf<double> pacc64;
+  spk::DevBuf<float> pacc32;
 
+  spk_csr_s() = default;
+  spk_csr_s(const spk_csr_s&) = delete;
+  spk_csr_s& operator=(const spk_csr_s&) = delete;
+  ~spk_csr_s() { drop_panels(); }
+  void drop_panels() {
+    for (auto* q : panels) delete q;
+    panels.clear();
+  }
   // allocates rp, ci and v64 (with the zeroed kPad tail)
   void allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_);
   const float* vals32();
   template <typename T>
   spk::CsrView<T> view();
+  template <typename T>
+  T* panel_acc();
 };
+
+template <>
+inline double* spk_csr_s::panel_acc<double>() {
+  if (pacc64.n < (size_t)nrows) pacc64.alloc((size_t)nrows);
+  return pacc64.p;
+}
+template <>
+inline float* spk_csr_s::panel_acc<float>() {
+  if (pacc32.n < (size_t)nrows) pacc32.alloc((size_t)nrows);
+  return pacc32.p;
+}
 
 template <>
 inline spk::CsrView<double> spk_csr_s::view<double>() {
@@ -84,6 +113,8 @@ void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks);
 // Finish a store whose rp/ci/v64 are filled on the device: copies rp back,
 // validates it and builds the plan.
 void finish_store(spk_csr_s& A, bool validate_cols);
+// Split A into `panels` column panels (1: none); values are copied.
+void build_panels(spk_csr_s& A, int panels);
 
 // ------------------------------------------------------------- epilogues
 //
@@ -133,9 +164,11 @@ struct Pair<float> {
 };
 typedef int IdxPair __attribute__((ext_vector_type(2)));
 
+// init (may be null): each row's sum starts from init[row] (column panels)
 template <typename T, typename Epi>
 __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
-    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials) {
+    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials,
+    const T* __restrict__ init) {
   __shared__ T prod[kNnzBlock + 2];
   __shared__ double red[kSpmvBlock / 64];
   if (epi.done && *epi.done) return;
@@ -150,7 +183,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
       s += __builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
     s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
-    if (tid == 0) acc = epi.row(r0, s);
+    if (tid == 0) acc = epi.row(r0, init ? init[r0] + s : s);
   } else {
     // lanes per row: enough rows to occupy the block, about eight products
     // per lane; short rows (stencils) get one lane each
@@ -197,7 +230,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     if (L == 1) {
       // storage-order sum (sparse_blas.hpp:26-31)
       if (g < nrows) {
-        T s = T(0);
+        T s = init ? init[r0 + g] : T(0);
         for (int j = s0; j < s1; ++j) s += prod[j];
         acc = epi.row(r0 + g, s);
       }
@@ -206,7 +239,8 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
       if (g < nrows)
         for (int j = s0 + lane; j < s1; j += L) s += prod[j];
       for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if (g < nrows && lane == 0) acc = epi.row(r0 + g, s);
+      if (g < nrows && lane == 0)
+        acc = epi.row(r0 + g, init ? init[r0 + g] + s : s);
     }
   }
   if constexpr (Epi::kReduce) {
@@ -216,11 +250,11 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
 }
 
 template <typename T, typename Epi>
-void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
-  if (A.nrows == 0) return;
+void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
+                     hipStream_t s) {
   double* parts = Epi::kReduce ? reduce_partials(A.nblocks) : nullptr;
   hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
-                     0, s, A.view<T>(), x, epi, parts);
+                     0, s, A.view<T>(), x, epi, parts, init);
   SPK_HIP_TRY(hipGetLastError());
   if constexpr (Epi::kReduce) {
     hipLaunchKernelGGL((finish_kernel<1, EpiFinalize<Epi>>), dim3(1),
@@ -228,6 +262,25 @@ void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
                        A.nblocks, EpiFinalize<Epi>{epi}, epi.done);
     SPK_HIP_TRY(hipGetLastError());
   }
+}
+
+template <typename T, typename Epi>
+void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
+  if (A.nrows == 0) return;
+  if (A.panels.empty()) {
+    launch_spmv_one(A, x, epi, (const T*)nullptr, s);
+    return;
+  }
+  T* acc = A.panel_acc<T>();
+  const T* init = nullptr;
+  for (size_t p = 0; p + 1 < A.panels.size(); ++p) {
+    EpiStore<T> part;
+    part.y = acc;
+    part.done = epi.done;
+    launch_spmv_one(*A.panels[p], x, part, init, s);
+    init = acc;
+  }
+  launch_spmv_one(*A.panels.back(), x, epi, init, s);
 }
 
 }  // namespace spk

@@ -13,6 +13,12 @@
 //                    block runs the Givens rotations and the convergence test
 //                    on the device (FP64), then V_{j+1} is normalised
 //   form_solution:   back-substitution (one thread) and x += Z y
+// Optional orthogonalization SPK_ORTHO_CGS2 (classical Gram-Schmidt applied
+// twice): per step two passes of {all j+1 dot products in one launch (a
+// chunk x basis-vector grid), one fixed-order finisher, one fused update
+// w -= sum_i h_i V_i}, the second update fused with |w|^2 -- nine launches
+// per step whatever j is, instead of the 2(j+2)+1 of the sequential MGS
+// sweep; each update runs in FP64 and rounds once.
 // The host reads the state once per cycle.  Vectors may be FP32
 // (precision = SPK_F32); H, the rotations, g, y, every dot product and the
 // Gram-Schmidt updates are computed in FP64 and the basis is stored in T.
@@ -83,6 +89,69 @@ __device__ void arnoldi_finish(GmState* st, int j, double h) {
   }
 }
 
+constexpr int kDotChunk = 4096;  // elements per (chunk, vector) block
+
+// partial <V_i, w> over one chunk: grid (chunks, j + 1)
+template <typename T>
+__global__ __launch_bounds__(256) void multi_dot_kernel(
+    int64_t n, const T* __restrict__ V, size_t ldv, const T* __restrict__ w,
+    int nchunks, double* __restrict__ partials, const int* stop) {
+  __shared__ double red[4];
+  if (*stop) return;
+  const int i = blockIdx.y, c = blockIdx.x;
+  const T* __restrict__ vi = V + (size_t)i * ldv;
+  const int64_t e0 = (int64_t)c * kDotChunk;
+  const int64_t e1 = e0 + kDotChunk < n ? e0 + kDotChunk : n;
+  double acc = 0.0;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256)
+    acc += (double)vi[e] * (double)w[e];
+  acc = block_reduce_sum<double, 256>(acc, red);
+  if (threadIdx.x == 0) partials[(size_t)i * nchunks + c] = acc;
+}
+
+// coef[i] = sum of the chunk partials (fixed order); also added to add_to[i]
+__global__ __launch_bounds__(256) void multi_finish_kernel(
+    int nchunks, const double* __restrict__ partials, double* __restrict__ coef,
+    double* __restrict__ add_to, const int* stop) {
+  __shared__ double red[4];
+  if (*stop) return;
+  const int i = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nchunks; k += 256)
+    acc += partials[(size_t)i * nchunks + k];
+  acc = block_reduce_sum<double, 256>(acc, red);
+  if (threadIdx.x == 0) {
+    coef[i] = acc;
+    if (add_to) add_to[i] += acc;
+  }
+}
+
+// w -= sum_{i <= j} coef[i] V_i in FP64, one rounding; NORM: partial |w|^2
+template <typename T, bool NORM>
+__global__ __launch_bounds__(256) void multi_update_kernel(
+    int64_t n, const T* __restrict__ V, size_t ldv, int jp1,
+    const double* __restrict__ coef, T* __restrict__ w,
+    double* __restrict__ partials, const int* stop) {
+  extern __shared__ double hc[];
+  __shared__ double red[4];
+  if (*stop) return;
+  for (int i = threadIdx.x; i < jp1; i += 256) hc[i] = coef[i];
+  __syncthreads();
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * 256) {
+    double s = (double)w[e];
+    for (int i = 0; i < jp1; ++i) s -= hc[i] * (double)V[(size_t)i * ldv + e];
+    const T t = (T)s;
+    w[e] = t;
+    if (NORM) acc += (double)t * (double)t;
+  }
+  if (NORM) {
+    acc = block_reduce_sum<double, 256>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+  }
+}
+
 // form_solution's back-substitution R y = g (gmres.hpp:333-342)
 __global__ void back_substitute_kernel(GmState* st) {
   if (st->done || threadIdx.x != 0) return;
@@ -118,6 +187,19 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
                       ? (double*)ws.get(kWsGmres + 4, sizeof(double) *
                                         (size_t)std::max<int64_t>(cfg->max_iterations, 1))
                       : nullptr;
+  SPK_REQUIRE(cfg->orthogonalization == SPK_ORTHO_MGS ||
+                  cfg->orthogonalization == SPK_ORTHO_CGS2,
+              SPK_INVALID_ARGUMENT, "unknown orthogonalization");
+  const bool cgs2 = cfg->orthogonalization == SPK_ORTHO_CGS2;
+  SPK_REQUIRE(!cgs2 || m < 4096, SPK_INVALID_ARGUMENT,
+              "classical Gram-Schmidt supports restart dimensions below 4096");
+  double* ctmp = (double*)ws.get(kWsGmres + 6, sizeof(double) * (size_t)(m + 1));
+  double* parts = nullptr;
+  if (cgs2) {  // sized once: grows (and synchronises) only here
+    const size_t nch = (size_t)((n + kDotChunk - 1) / kDotChunk);
+    parts = reduce_partials(std::max<size_t>((size_t)(m + 1) * nch,
+                                             (size_t)vec_grid(n)));
+  }
   DevState<GmState> st(kWsGmres + 5);
   GmState h{};
   h.tol = cfg->tolerance;
@@ -197,6 +279,34 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
         launch_spmv(*A, (const T*)zj, EpiStore<T>{r0, stop}, s);
         left->apply((const T*)r0, vj1);
       }
+      if (cgs2) {
+        double* hcol = small + j * ld;
+        const int jp1 = (int)(j + 1);
+        const int nch = (int)((n + kDotChunk - 1) / kDotChunk);
+        const size_t dyn = sizeof(double) * (size_t)jp1;
+        for (int pass = 0; pass < 2; ++pass) {
+          hipLaunchKernelGGL((multi_dot_kernel<T>), dim3(nch, jp1), dim3(256), 0, s,
+                             n, (const T*)V, nn, (const T*)vj1, nch, parts, stop);
+          hipLaunchKernelGGL(multi_finish_kernel, dim3(jp1), dim3(256), 0, s, nch,
+                             (const double*)parts, pass == 0 ? hcol : ctmp,
+                             pass == 0 ? (double*)nullptr : hcol, stop);
+          if (pass == 0) {
+            hipLaunchKernelGGL((multi_update_kernel<T, false>), dim3(vec_grid(n)),
+                               dim3(256), dyn, s, n, (const T*)V, nn, jp1,
+                               (const double*)hcol, vj1, parts, stop);
+          } else {
+            const int g = vec_grid(n);
+            hipLaunchKernelGGL((multi_update_kernel<T, true>), dim3(g), dim3(256),
+                               dyn, s, n, (const T*)V, nn, jp1,
+                               (const double*)ctmp, vj1, parts, stop);
+            const int jj = (int)j;
+            launch_finish<1>(parts, g, [=] __device__(const double* t) {
+              arnoldi_finish(S, jj, sqrt(t[0]));
+            }, stop);
+          }
+          SPK_HIP_TRY(hipGetLastError());
+        }
+      } else {
       // modified Gram-Schmidt (gmres.hpp:258-267)
       for (int64_t i = 0; i <= j; ++i) {
         const T* vi = V + (size_t)i * nn;
@@ -223,6 +333,7 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
           arnoldi_finish(S, jj, sqrt(t[0]));
         }, stop);
       }
+      }  // MGS
       // normalise V_{j+1} if h > 0 (gmres.hpp:276-280 divides,
       // fgmres.hpp:283 multiplies by the reciprocal)
       if (flexible) {

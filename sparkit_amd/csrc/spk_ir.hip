@@ -64,6 +64,7 @@ void ir_solve(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   inner.max_iterations = cfg->max_inner;
   inner.collect_residuals = 0;
   inner.precision = SPK_F32;
+  inner.orthogonalization = cfg->orthogonalization;
   for (int64_t k = 0;; ++k) {
     double rel = h.rnorm / h.bnorm;
     if (history) history[k] = rel;

@@ -273,6 +273,41 @@ void test_errors() {
   }));
 }
 
+void test_learned_spai_and_batched() {
+  // a 9-point anisotropic diffusion matrix from the device generator
+  spk_csr g = nullptr;
+  sparkit::hip::check(spk_gen_aniso_q1_2d(24, 5, 1.5, 0, &g), "gen");
+  sh::Device_csr A(g);
+  sh::Learned_spai net(3);
+  const double l0 = net.loss(A);
+  std::vector<sh::Device_csr> mats;
+  for (int s = 0; s < 4; ++s) {
+    spk_csr h = nullptr;
+    sparkit::hip::check(spk_gen_aniso_q1_2d(24, 100 + s, 1.5, 0, &h), "gen");
+    mats.emplace_back(h);
+  }
+  auto hist = net.train(mats, 6, 2e-3);
+  CHECK(hist.size() == 24);
+  CHECK(net.loss(A) < l0);
+  auto M = net.predict(A);
+  std::vector<double> bh(A.rows(), 1.0), xh(A.rows(), 0.0);
+  sh::Gmres_config<double> cfg{1e-9, 30, 2000, false};
+  auto s = sh::fgmres(bh.begin(), bh.end(), xh.begin(), xh.end(), cfg, A, M);
+  CHECK(s.converged);
+  CHECK(throws<std::invalid_argument>([&] {
+    net.set_parameters(std::vector<float>(3, 0.f));
+  }));
+  // batched PCG: two systems
+  auto B = sh::Batched_pcg::diffusion(5, 9, 4, 12);
+  CHECK(B.count() == 5);
+  std::vector<double> ones(B.rows(), 1.0), zeros(B.rows(), 0.0);
+  sh::Device_vector<double> bb(ones.begin(), ones.end()), xb(zeros.begin(), zeros.end());
+  spk_batch_config bc{1e-10, 1e-4, 20, 1000, SPK_F32, 0};
+  auto bs = B.solve(bc, bb, xb);
+  CHECK(bs.converged == 5);
+  CHECK(bs.max_relres < 1e-10);
+}
+
 }  // namespace
 
 int main() {
@@ -284,7 +319,8 @@ int main() {
                {"cg_cases", test_cg_cases},
                {"gmres_fgmres_bicgstab", test_gmres_fgmres_bicgstab},
                {"device_vectors_and_fp32", test_device_vectors_and_fp32},
-               {"errors", test_errors}};
+               {"errors", test_errors},
+               {"learned_spai_and_batched", test_learned_spai_and_batched}};
   for (const auto& c : cases) {
     int before = g_failures;
     try {

@@ -1,0 +1,121 @@
+"""Batched persistent Jacobi-PCG on the GPU: the device-built batch is
+bit-identical to the oracle's systems; every system reaches the FP64 true
+residual tolerance (checked independently by the oracle's SpMV); the FP64
+mode's per-system iteration counts match the oracle's Jacobi-PCG within one;
+graph replays and the autotuner's launch shapes give identical results."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED, SMIN, SMAX, COUNT = 77, 6, 40, 48
+
+
+@pytest.fixture(scope="module")
+def batch(spk):
+    return spk.BatchedSystems.diffusion(COUNT, SEED, SMIN, SMAX)
+
+
+def sys_csr(O, rp, ci, v, r0, r1):
+    a, b = rp[r0], rp[r1]
+    return O.Csr(r1 - r0, r1 - r0, rp[r0:r1 + 1] - a, ci[a:b] - r0,
+                 v[a:b].copy())
+
+
+def test_generator_bit_exact(O, spk, batch):
+    ro = batch.row_offsets()
+    rp, ci, v = batch.matrix.download()
+    assert batch.count == COUNT and ro[-1] == batch.total_rows
+    for k in range(COUNT):
+        R = O.batched_diffusion(SEED, k, SMIN, SMAX)
+        S = sys_csr(O, rp, ci, v, ro[k], ro[k + 1])
+        assert np.array_equal(S.row_ptr, R.row_ptr), k
+        assert np.array_equal(S.col_ind, R.col_ind), k
+        assert np.array_equal(S.vals.view(np.uint64), R.vals.view(np.uint64)), k
+
+
+def _rhs(spk, batch):
+    return spk.uniform_pm1(5, batch.total_rows)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_every_system_converges(O, spk, batch, prec):
+    b = _rhs(spk, batch)
+    x = spk.DeviceVector.zeros(batch.total_rows)
+    if prec == "f32":
+        cfg = spk.BatchConfig(1e-8, 1e-4, 30, 5000, spk.F32)
+    else:
+        cfg = spk.BatchConfig(1e-8, 1e-8, 1, 5000, spk.F64)
+    s = batch.solve(cfg, b, x)
+    assert s["converged"] == COUNT and s["max_relres"] < 1e-8
+    it, ou, rr, cv = batch.results()
+    assert np.all(cv == 1) and np.all(rr < 1e-8)
+    ro = batch.row_offsets()
+    rp, ci, v = batch.matrix.download()
+    bh, xh = b.numpy(), x.numpy()
+    for k in range(0, COUNT, 5):
+        S = sys_csr(O, rp, ci, v, ro[k], ro[k + 1])
+        bk, xk = bh[ro[k]:ro[k + 1]], xh[ro[k]:ro[k + 1]]
+        rel = O.true_relres(S, bk, xk)
+        assert rel < 1e-8
+        assert abs(rel - rr[k]) <= 1e-3 * rel + 1e-15
+        if prec == "f64":
+            xr, sr = O.cg(S, bk, tol=1e-8, restart_iterations=5000,
+                          max_iterations=5000, left="jacobi")
+            # the device judges on the true residual, the reference on the
+            # recursive one: equal up to a step either way
+            assert abs(it[k] - sr.computed_iterations) <= 2, k
+            assert np.linalg.norm(xk - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
+def test_graph_replay_and_tunings_agree(spk, batch):
+    b = _rhs(spk, batch)
+    cfg = spk.BatchConfig(1e-8, 1e-4, 30, 5000, spk.F32)
+    x0 = spk.DeviceVector.zeros(batch.total_rows)
+    batch.tuning = (256, 2, 0)
+    batch.solve(cfg, b, x0)
+    ref = x0.numpy()
+    it0 = batch.results()[0].copy()
+    x1 = spk.DeviceVector.zeros(batch.total_rows)
+    for rep in range(3):  # capture, then replays
+        batch.solve(cfg, b, x1, use_graph=True)
+        assert np.array_equal(x1.numpy(), ref), rep
+        assert np.array_equal(batch.results()[0], it0)
+    for t in [(512, 1, 4096), (1024, 2, 0), (256, 4, 16384)]:
+        batch.tuning = t
+        x2 = spk.DeviceVector.zeros(batch.total_rows)
+        batch.solve(cfg, b, x2)
+        # same arithmetic per system except the reduction tree width
+        assert np.all(batch.results()[3] == 1)
+        assert np.linalg.norm(x2.numpy() - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
+def test_autotune(spk, batch):
+    b = _rhs(spk, batch)
+    x = spk.DeviceVector.zeros(batch.total_rows)
+    cfg = spk.BatchConfig(1e-8, 1e-4, 30, 5000, spk.F32)
+    chosen, tried = batch.autotune(cfg, b, x, sample=16)
+    assert len(tried) >= 10 and chosen in [t for t, _ in tried]
+    assert min(ms for _, ms in tried) == dict(tried)[chosen]
+    assert batch.tuning == chosen
+    s = batch.solve(cfg, b, x)
+    assert s["converged"] == COUNT
+
+
+def test_from_blocks_and_errors(O, spk):
+    A1 = O.batched_diffusion(3, 0, 4, 4)
+    A2 = O.tridiagonal_matrix(5, -1.0, 3.0, -1.0)
+    n1, n2 = A1.nrows, A2.nrows
+    rp = np.concatenate([A1.row_ptr, A2.row_ptr[1:] + A1.nnz])
+    ci = np.concatenate([A1.col_ind, A2.col_ind + n1])
+    v = np.concatenate([A1.vals, A2.vals])
+    A = spk.CsrMatrix.from_host(n1 + n2, n1 + n2, rp, ci, v)
+    B = spk.BatchedSystems.from_blocks(A, [0, n1, n1 + n2])
+    b = spk.DeviceVector.from_numpy(np.ones(n1 + n2))
+    x = spk.DeviceVector.zeros(n1 + n2)
+    s = B.solve(spk.BatchConfig(1e-10, 1e-10, 1, 100, spk.F64), b, x)
+    assert s["converged"] == 2
+    with pytest.raises(spk.InvalidArgument):   # block 0 would couple to 1
+        spk.BatchedSystems.from_blocks(A, [0, n1 - 1, n1 + n2])
+    with pytest.raises(spk.InvalidArgument):
+        B.tuning = (300, 1, 0)
