@@ -586,4 +586,67 @@ Summary<T> bicgstab(BIter bfirst, BIter blast, XIter xfirst, XIter xlast,
                              });
 }
 
+// cgs (cgs.hpp:383-395); Cgs_config has Bicgstab_config's fields
+template <typename T>
+using Cgs_config = Bicgstab_config<T>;
+template <typename Config, typename Left, typename Right,
+          typename T = detail::value_of<Config>>
+Summary<T> cgs(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
+               const Device_csr& A, const Left& M_left, const Right& M_right) {
+  auto c = detail::make_config<T>(cfg.tolerance, 0, cfg.max_iterations,
+                                  cfg.collect_residuals);
+  return detail::run_device<T>(c, [&](const spk_solver_config* pc,
+                                      spk_summary* s, double* h) {
+    return spk_cgs_solve(A.handle(), detail::slot(M_left), detail::slot(M_right),
+                         pc, b.data(), x.data(), s, h);
+  });
+}
+
+template <typename BIter, typename XIter, typename Config, typename Left,
+          typename Right, typename T = detail::value_of<Config>>
+Summary<T> cgs(BIter bfirst, BIter blast, XIter xfirst, XIter xlast, Config cfg,
+               const Device_csr& A, const Left& M_left, const Right& M_right) {
+  return detail::run_host<T>(bfirst, blast, xfirst, xlast,
+                             [&](Device_vector<T>& b, Device_vector<T>& x) {
+                               return cgs(b, x, cfg, A, M_left, M_right);
+                             });
+}
+
+// A^T as a store: the transpose operator of bicg (multiply_transpose,
+// sparse_blas.hpp:36-52)
+inline Device_csr transpose(const Device_csr& A) {
+  spk_csr t = nullptr;
+  check(spk_csr_transpose(A.handle(), &t), "spk_csr_transpose");
+  return Device_csr(t);
+}
+
+// bicg (bicg.hpp:86-397): the transpose operator slot takes the store of
+// A^T; Bicg_config has Bicgstab_config's fields
+template <typename T>
+using Bicg_config = Bicgstab_config<T>;
+template <typename Config, typename Left, typename Right,
+          typename T = detail::value_of<Config>>
+Summary<T> bicg(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
+                const Device_csr& A, const Device_csr& At, const Left& M_left,
+                const Right& M_right) {
+  auto c = detail::make_config<T>(cfg.tolerance, 0, cfg.max_iterations,
+                                  cfg.collect_residuals);
+  return detail::run_device<T>(c, [&](const spk_solver_config* pc,
+                                      spk_summary* s, double* h) {
+    return spk_bicg_solve(A.handle(), At.handle(), detail::slot(M_left),
+                          detail::slot(M_right), pc, b.data(), x.data(), s, h);
+  });
+}
+
+template <typename BIter, typename XIter, typename Config, typename Left,
+          typename Right, typename T = detail::value_of<Config>>
+Summary<T> bicg(BIter bfirst, BIter blast, XIter xfirst, XIter xlast, Config cfg,
+                const Device_csr& A, const Device_csr& At, const Left& M_left,
+                const Right& M_right) {
+  return detail::run_host<T>(bfirst, blast, xfirst, xlast,
+                             [&](Device_vector<T>& b, Device_vector<T>& x) {
+                               return bicg(b, x, cfg, A, At, M_left, M_right);
+                             });
+}
+
 }  // namespace sparkit::hip

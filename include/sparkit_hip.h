@@ -119,6 +119,10 @@ spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y);
 spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
                             const int64_t* d_rows, const int64_t* d_cols,
                             const double* d_vals, spk_csr* out);
+/* A^T as a new store (the assembly of the swapped entries): its SpMV is
+ * multiply_transpose(A, x) (sparse_blas.hpp:36-52) with each output summed
+ * in the reference's order. */
+spk_status spk_csr_transpose(spk_csr A, spk_csr* out);
 
 /* Permutations on the device, perm (device, int64, perm[old] = new) of
  * nrows (rows, both) or ncols (columns) entries: rperm (mode
@@ -176,6 +180,17 @@ spk_status spk_precond_create_jacobi(spk_csr A, spk_precond* out);
  * U its diagonal first (ilu.hpp:123-141).  SPK_INVALID_ARGUMENT for a
  * non-square matrix or a missing diagonal. */
 spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out);
+/* Schedule of the ILU(0) triangular sweeps (no reference counterpart; the
+ * arithmetic, and so the result, is the same for both): SPK_ILU_AUTO (0,
+ * the default: per-level launches for at least 65536 rows and at most 4096
+ * levels, else sync-free), SPK_ILU_LEVELS (1: one launch per dependency
+ * level), SPK_ILU_SYNC_FREE (2: one launch per sweep, rows wait for their
+ * dependencies' published values). */
+#define SPK_ILU_AUTO 0
+#define SPK_ILU_LEVELS 1
+#define SPK_ILU_SYNC_FREE 2
+spk_status spk_ilu0_set_schedule(spk_precond M, int schedule);
+spk_status spk_ilu0_get_schedule(spk_precond M, int* schedule);
 /* Identity slot (conjugate_gradient_test.cpp:45-47). */
 spk_status spk_precond_create_identity(int64_t n, spk_precond* out);
 /* z = M r for an explicit sparse approximate inverse M (a store). */
@@ -256,6 +271,21 @@ spk_status spk_bicgstab_solve(spk_csr A, spk_precond M_left,
                               const spk_solver_config* cfg, const void* d_b,
                               void* d_x, spk_summary* summary,
                               double* history);
+
+/* cgs(b, x, cfg, A, M_L, M_R) (cgs.hpp:383-395); counts matvecs (two per
+ * step); a zero <r^, q^> or rho stops with SPK_BREAKDOWN (undetected in the
+ * reference).  cfg->restart is ignored. */
+spk_status spk_cgs_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
+                         const spk_solver_config* cfg, const void* d_b,
+                         void* d_x, spk_summary* summary, double* history);
+
+/* bicg(b, x, cfg, A, A^T, M_L, M_R) (bicg.hpp:86-397): the transpose
+ * operator is the store At (spk_csr_transpose(A)); counts matvecs (two per
+ * step); zero <p~, q> or rho stop with SPK_BREAKDOWN. */
+spk_status spk_bicg_solve(spk_csr A, spk_csr At, spk_precond M_left,
+                          spk_precond M_right, const spk_solver_config* cfg,
+                          const void* d_b, void* d_x, spk_summary* summary,
+                          double* history);
 
 /* Mixed-precision iterative refinement (no reference counterpart; the
  * closest structure is the outer correction loop of richardson.hpp:112):
@@ -377,27 +407,29 @@ spk_status spk_spai_predict(spk_spai_model m, spk_csr A, spk_csr* M_out);
  * Thousands of independent SPD systems, stored as the diagonal blocks of
  * one CSR store, solved by Jacobi-PCG in one persistent launch: each
  * workgroup takes the next system (largest first) from a device queue and
- * runs, until ||b - A x|| / ||b|| < tolerance in FP64, refinement steps of
- * an inner Jacobi-PCG in `precision` (SPK_F32: FP32 SpMV and vectors,
- * FP64 dot products) to inner_tolerance on the scaled residual, with an
- * FP64 true-residual check after each step.  SPK_F64 with max_outer = 1 and
- * inner_tolerance = tolerance is plain FP64 Jacobi-PCG per system
- * (conjugate_gradient.hpp:122-312 with jacobi.hpp:25-50) whose convergence
- * is judged on the true residual.  No reference counterpart for the
- * batching itself. */
+ * runs Jacobi-PCG in `precision` (SPK_F32: FP32 SpMV and vectors, FP64 dot
+ * products) with reliable updates: whenever the recurrence residual has
+ * dropped by inner_tolerance since the last update, the FP32 correction is
+ * added to x in FP64 and the FP64 true residual b - A x replaces the
+ * recurrence residual (the search direction is kept).  A system is done
+ * when its FP64 true relative residual is below tolerance.  SPK_F64 is
+ * plain FP64 Jacobi-PCG per system (conjugate_gradient.hpp:122-312 with
+ * jacobi.hpp:25-50) judged on the true residual.  No reference
+ * counterpart for the batching itself. */
 typedef struct spk_batch_s* spk_batch;
 typedef struct {
   double tolerance;        /* FP64 true relative residual per system */
-  double inner_tolerance;  /* inner solve, relative to its right-hand side */
-  int64_t max_outer;
-  int64_t max_inner;       /* per refinement step */
+  double inner_tolerance;  /* residual reduction between FP64 updates */
+  int64_t max_outer;       /* FP64 true-residual updates per system */
+  int64_t max_inner;       /* PCG iterations per system */
   int32_t precision;       /* SPK_F32 or SPK_F64 inner arithmetic */
   int32_t pad;
 } spk_batch_config;
 typedef struct {
   int32_t block;      /* threads per workgroup: 256, 512, 1024 */
   int32_t wg_per_cu;  /* persistent workgroups per CU */
-  int32_t lds_rows;   /* systems up to this many rows keep p in LDS */
+  int32_t lds_rows;   /* systems up to this many rows keep p in LDS (capped
+                         to 128 KB per CU at the solve's precision) */
   int32_t pad;
 } spk_batch_tuning;
 typedef struct {
@@ -424,9 +456,10 @@ spk_status spk_batch_info(spk_batch B, int64_t* count, int64_t* total_rows,
 spk_status spk_batch_row_offsets(spk_batch B, int64_t* row_offsets);
 spk_status spk_batch_set_tuning(spk_batch B, const spk_batch_tuning* t);
 spk_status spk_batch_get_tuning(spk_batch B, spk_batch_tuning* t);
-/* times every candidate launch shape on `sample` systems spread over the
- * size range (b and x are used as scratch: x is overwritten), keeps the
- * fastest; tried / tried_ms (may be NULL, room for 36) list the candidates */
+/* times every candidate launch shape (256/512/1024 threads x 1/2/4
+ * workgroups per CU x LDS bin threshold 0/16384 rows, those that fit) on
+ * `sample` systems spread over the size range (x is overwritten), keeps
+ * the fastest; tried / tried_ms (may be NULL, room for 18) list them */
 spk_status spk_batch_autotune(spk_batch B, const spk_batch_config* cfg,
                               const double* d_b, double* d_x, int64_t sample,
                               spk_batch_tuning* chosen, spk_batch_tuning* tried,

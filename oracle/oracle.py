@@ -109,7 +109,7 @@ def _declare(L):
     L.orc_splitmix64_at.argtypes = [C.c_uint64, C.c_uint64]
     L.orc_splitmix64_at.restype = C.c_uint64
     L.orc_uniform_pm1.argtypes = [C.c_uint64, C.c_int64, _f64p]
-    for nm in ("orc_cg", "orc_gmres", "orc_bicgstab"):
+    for nm in ("orc_cg", "orc_gmres", "orc_bicgstab", "orc_cgs"):
         for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
                                    ("f", _f32p, ConfigF, SummaryF)):
             getattr(L, f"{nm}_{sfx}").argtypes = [
@@ -155,6 +155,13 @@ def _declare(L):
                                       C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
     L.orc_gen_aniso_q1_2d.restype = C.c_int64
+    L.orc_spmv_transpose_d.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p,
+                                       _f64p, _f64p, _f64p]
+    for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
+                               ("f", _f32p, ConfigF, SummaryF)):
+        getattr(L, f"orc_bicg_{sfx}").argtypes = [
+            C.c_int64, fp, fp, C.POINTER(cfg)] + [C.c_void_p] * 8 + [
+            C.POINTER(summ), C.c_void_p]
     L.orc_is_valid_permutation.argtypes = [C.c_int64, _i64p]
     L.orc_csr_permute.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p,
                                   C.c_void_p, _i64p, C.c_int, _i64p, _i64p,
@@ -374,12 +381,13 @@ class _Op:
             self.ctx = None
             return
         kind, data = spec if isinstance(spec, tuple) else (spec, None)
-        if kind == "csr":
+        if kind in ("csr", "csrT"):
             M = data if data is not None else A
             v = np.ascontiguousarray(M.vals, dtype)
             s = M._struct(v)
             self.keep += [v, s, M]
-            self.fn = C.cast(getattr(L, f"orc_apply_csr_{sfx}"), C.c_void_p)
+            nm = "orc_apply_csr" if kind == "csr" else "orc_apply_csr_transpose"
+            self.fn = C.cast(getattr(L, f"{nm}_{sfx}"), C.c_void_p)
             self.ctx = C.cast(C.pointer(s), C.c_void_p)
         elif kind == "jacobi":
             inv_d = jacobi(A) if data is None else data
@@ -414,7 +422,13 @@ def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
     hist = np.zeros(max(max_iterations, 1), dtype)
     a_op = _Op(("csr", A), A, dtype)
     ml = _Op(left, A, dtype)
-    if name == "fgmres":
+    if name == "bicg":
+        at = _Op(("csrT", A), A, dtype)
+        mr = _Op(right, A, dtype)
+        getattr(L, f"orc_bicg_{sfx}")(n, b, x, C.byref(cfg), a_op.fn, a_op.ctx,
+                                      at.fn, at.ctx, ml.fn, ml.ctx, mr.fn,
+                                      mr.ctx, C.byref(summ), _ptr(hist))
+    elif name == "fgmres":
         getattr(L, f"orc_fgmres_{sfx}")(n, b, x, C.byref(cfg), a_op.fn,
                                         a_op.ctx, ml.fn, ml.ctx,
                                         C.byref(summ), _ptr(hist))
@@ -448,6 +462,27 @@ def fgmres(A, b, x0=None, *, tol, restart_dimension, max_iterations,
     """fgmres(): fgmres.hpp:96-470."""
     return _solve("fgmres", A, b, x0, tol, restart_dimension, max_iterations,
                   preconditioner, None, dtype)
+
+
+def bicg(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
+         dtype=np.float64):
+    """bicg(): bicg.hpp:86-397, transpose operator multiply_transpose."""
+    return _solve("bicg", A, b, x0, tol, 0, max_iterations, left, right, dtype)
+
+
+def spmv_transpose(A, x):
+    """multiply_transpose(A, x): sparse_blas.hpp:36-52."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(A.ncols, np.float64)
+    lib().orc_spmv_transpose_d(A.nrows, A.ncols, A.row_ptr, A.col_ind,
+                               np.ascontiguousarray(A.vals, np.float64), x, y)
+    return y
+
+
+def cgs(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
+        dtype=np.float64):
+    """cgs(): cgs.hpp:86-395."""
+    return _solve("cgs", A, b, x0, tol, 0, max_iterations, left, right, dtype)
 
 
 def bicgstab(A, b, x0=None, *, tol, max_iterations, left=None, right=None,

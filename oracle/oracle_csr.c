@@ -306,6 +306,26 @@ ORC_TRISOLVE(f, float)
 ORC_OPS(d, double, orc_csr_d, orc_ilu_d)
 ORC_OPS(f, float, orc_csr_f, orc_ilu_f)
 
+/* multiply_transpose(A, x): sparse_blas.hpp:36-52 -- y = 0, then for each row
+ * i in order and each entry (i, j) in storage order y_j += a_ij x_i */
+#define ORC_SPMV_T(SFX, T, CSR)                                                \
+  void orc_spmv_transpose_##SFX(int64_t nrows, int64_t ncols,                  \
+                                const int64_t* rp, const int64_t* ci,          \
+                                const T* v, const T* x, T* y) {                \
+    for (int64_t j = 0; j < ncols; ++j) y[j] = (T)0;                           \
+    for (int64_t i = 0; i < nrows; ++i)                                        \
+      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) y[ci[p]] += v[p] * x[i];     \
+  }                                                                            \
+  void orc_apply_csr_transpose_##SFX(const void* ctx, int64_t n, const T* in,  \
+                                     T* out) {                                 \
+    const CSR* A = (const CSR*)ctx;                                            \
+    (void)n;                                                                   \
+    orc_spmv_transpose_##SFX(A->nrows, A->ncols, A->row_ptr, A->col_ind,       \
+                             A->vals, in, out);                                \
+  }
+ORC_SPMV_T(d, double, orc_csr_d)
+ORC_SPMV_T(f, float, orc_csr_f)
+
 double orc_true_relres_d(int64_t n, const int64_t* rp, const int64_t* ci,
                          const double* v, const double* b, const double* x) {
   double rr = 0.0, bb = 0.0;

@@ -166,6 +166,17 @@ void orc_apply_identity_f(const void* ctx, int64_t n, const float* in, float* ou
 /* operator lambda over multiply(A, x): conjugate_gradient_test.cpp:61-64 */
 void orc_apply_csr_d(const void* ctx, int64_t n, const double* in, double* out);
 void orc_apply_csr_f(const void* ctx, int64_t n, const float* in, float* out);
+/* multiply_transpose(A, x): sparse_blas.hpp:36-52 */
+void orc_spmv_transpose_d(int64_t nrows, int64_t ncols, const int64_t* rp,
+                          const int64_t* ci, const double* v, const double* x,
+                          double* y);
+void orc_spmv_transpose_f(int64_t nrows, int64_t ncols, const int64_t* rp,
+                          const int64_t* ci, const float* v, const float* x,
+                          float* y);
+void orc_apply_csr_transpose_d(const void* ctx, int64_t n, const double* in,
+                               double* out);
+void orc_apply_csr_transpose_f(const void* ctx, int64_t n, const float* in,
+                               float* out);
 /* jacobi_apply lambda: jacobi_test.cpp:118-121; ctx points at inv_diag */
 void orc_apply_jacobi_d(const void* ctx, int64_t n, const double* in, double* out);
 void orc_apply_jacobi_f(const void* ctx, int64_t n, const float* in, float* out);
@@ -248,6 +259,26 @@ void orc_bicgstab_f(int64_t n, const float* b, float* x,
                     const orc_config_f* cfg, orc_op_f A, const void* actx,
                     orc_op_f ML, const void* mlctx, orc_op_f MR,
                     const void* mrctx, orc_summary_f* out, float* history);
+
+/* CgsSolver: cgs.hpp:86-395 (two matvecs per step, counted) */
+void orc_cgs_d(int64_t n, const double* b, double* x, const orc_config_d* cfg,
+               orc_op_d A, const void* actx, orc_op_d ML, const void* mlctx,
+               orc_op_d MR, const void* mrctx, orc_summary_d* out,
+               double* history);
+void orc_cgs_f(int64_t n, const float* b, float* x, const orc_config_f* cfg,
+               orc_op_f A, const void* actx, orc_op_f ML, const void* mlctx,
+               orc_op_f MR, const void* mrctx, orc_summary_f* out,
+               float* history);
+
+/* BicgSolver: bicg.hpp:86-397; At is the transpose operator */
+void orc_bicg_d(int64_t n, const double* b, double* x, const orc_config_d* cfg,
+                orc_op_d A, const void* actx, orc_op_d At, const void* atctx,
+                orc_op_d ML, const void* mlctx, orc_op_d MR, const void* mrctx,
+                orc_summary_d* out, double* history);
+void orc_bicg_f(int64_t n, const float* b, float* x, const orc_config_f* cfg,
+                orc_op_f A, const void* actx, orc_op_f At, const void* atctx,
+                orc_op_f ML, const void* mlctx, orc_op_f MR, const void* mrctx,
+                orc_summary_f* out, float* history);
 
 /* ||b - A x||_2 / ||b||_2 in double, sequential sums. */
 double orc_true_relres_d(int64_t n, const int64_t* rp, const int64_t* ci,

@@ -35,7 +35,9 @@ __all__ = [
     "identity", "jacobi", "ilu0", "milu0", "spai", "multiply",
     "extract_diagonal", "jacobi_apply", "ilu_apply", "CGConfig",
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
-    "conjugate_gradient", "gmres", "fgmres", "bicgstab", "true_relres",
+    "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
+    "bicg", "BicgConfig",
+    "true_relres",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB",
@@ -192,6 +194,12 @@ class CsrMatrix:
         call("spk_csr_plan_info", self.h, C.byref(nb), C.byref(nl))
         return nb.value, nl.value
 
+    def transpose(self) -> "CsrMatrix":
+        """A^T on the device; its SpMV is multiply_transpose(A, x)."""
+        h = C.c_void_p()
+        call("spk_csr_transpose", self.h, C.byref(h))
+        return CsrMatrix(h)
+
     def permute(self, perm, mode="both") -> "CsrMatrix":
         """rperm ('rows'), cperm ('cols') or dperm ('both', P A P^T) of
         permutation.hpp:41-148 (perm[old] = new), on the device."""
@@ -310,6 +318,18 @@ class Ilu0(Preconditioner):
         a, b = C.c_int64(), C.c_int64()
         call("spk_ilu0_levels", self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    @property
+    def schedule(self) -> str:
+        """'levels' (one launch per dependency level) or 'sync-free'."""
+        v = C.c_int()
+        call("spk_ilu0_get_schedule", self.h, C.byref(v))
+        return "levels" if v.value == 1 else "sync-free"
+
+    @schedule.setter
+    def schedule(self, name: str):
+        call("spk_ilu0_set_schedule", self.h,
+             {"auto": 0, "levels": 1, "sync-free": 2}[name])
 
     def forward(self, b: DeviceVector, x: DeviceVector):
         call("spk_ilu0_forward", self.h, _code(b.dtype), b.ptr, x.ptr)
@@ -622,6 +642,30 @@ def fgmres(b, x, cfg: GmresConfig, A: CsrMatrix, M=identity) -> Summary:
     c = _cfg(cfg, cfg.restart_dimension)
     return _run("spk_fgmres_solve", (A.h, _h(M), C.byref(c), b.ptr, x.ptr),
                 cfg)
+
+
+CgsConfig = BicgstabConfig  # Cgs_config has the same fields (cgs.hpp:24-42)
+BicgConfig = BicgstabConfig  # Bicg_config has the same fields (bicg.hpp:24-42)
+
+
+def bicg(b, x, cfg: BicgstabConfig, A: CsrMatrix, At: CsrMatrix,
+         M_left=identity, M_right=identity) -> Summary:
+    """bicg() (bicg.hpp:86-397) with the transpose operator given as the
+    store At = A.transpose(); counts matrix-vector products."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, 0)
+    return _run("spk_bicg_solve", (A.h, At.h, _h(M_left), _h(M_right),
+                                   C.byref(c), b.ptr, x.ptr), cfg)
+
+
+def cgs(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
+        M_right=identity) -> Summary:
+    """cgs() (cgs.hpp:383-395); computed_iterations counts matrix-vector
+    products, two per step."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, 0)
+    return _run("spk_cgs_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
+                                  b.ptr, x.ptr), cfg)
 
 
 def bicgstab(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,

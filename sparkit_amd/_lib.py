@@ -142,6 +142,8 @@ _SIGS = {
     "spk_ilu0_factor_nnz": ([vp, P_i64, P_i64], None),
     "spk_ilu0_download": ([vp, vp, vp, vp, vp, vp, vp], None),
     "spk_ilu0_levels": ([vp, P_i64, P_i64], None),
+    "spk_ilu0_set_schedule": ([vp, i32], None),
+    "spk_ilu0_get_schedule": ([vp, C.POINTER(C.c_int)], None),
     "spk_ilu0_forward": ([vp, i32, vp, vp], None),
     "spk_ilu0_backward": ([vp, i32, vp, vp], None),
     "spk_cg_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
@@ -152,6 +154,11 @@ _SIGS = {
                           C.POINTER(Summary), vp], None),
     "spk_bicgstab_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                             C.POINTER(Summary), vp], None),
+    "spk_cgs_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                       C.POINTER(Summary), vp], None),
+    "spk_bicg_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                        C.POINTER(Summary), vp], None),
+    "spk_csr_transpose": ([vp, C.POINTER(vp)], None),
     "spk_true_relres": ([vp, vp, vp, P_dbl], None),
     "spk_ir_solve": ([vp, vp, C.POINTER(IrConfig), vp, vp,
                       C.POINTER(IrSummary), vp], None),

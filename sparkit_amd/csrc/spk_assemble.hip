@@ -141,10 +141,44 @@ spk_csr_s* assemble(int64_t nrows, int64_t ncols, int64_t m, const int64_t* rows
   return A;
 }
 
+// (row, col) of every entry as int64 lists: the transpose is the assembly
+// of the swapped pairs
+__global__ void expand_kernel(int64_t n, const int32_t* __restrict__ rp,
+                              const int32_t* __restrict__ ci,
+                              int64_t* __restrict__ rows,
+                              int64_t* __restrict__ cols) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  for (int k = rp[i] + lane; k < rp[i + 1]; k += 64) {
+    rows[k] = i;
+    cols[k] = ci[k];
+  }
+}
+
+spk_csr_s* transpose(spk_csr_s& A) {
+  hipStream_t s = stream();
+  const int64_t m = A.nnz > 0 ? A.nnz : 1;
+  DevBuf<int64_t> r((size_t)m), c((size_t)m);
+  if (A.nnz > 0)
+    hipLaunchKernelGGL(expand_kernel, dim3(nb(A.nrows * 64)), dim3(256), 0, s,
+                       A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                       r.p, c.p);
+  SPK_HIP_TRY(hipGetLastError());
+  return assemble(A.ncols, A.nrows, A.nnz, c.p, r.p, A.v64.p);
+}
+
 }  // namespace
 }  // namespace spk
 
 using namespace spk;
+
+extern "C" spk_status spk_csr_transpose(spk_csr A, spk_csr* out) {
+  return guarded([&] {
+    SPK_REQUIRE(A && out, SPK_INVALID_ARGUMENT, "null argument");
+    *out = transpose(*A);
+  });
+}
 
 extern "C" spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
                                        const int64_t* d_rows, const int64_t* d_cols,

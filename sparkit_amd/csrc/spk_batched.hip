@@ -9,12 +9,15 @@
 // Kernel: every workgroup pulls the next system from a queue (a global
 // counter, systems ordered by decreasing size so the long solves start
 // first) and solves it alone:
-//   r = b - A x (FP64), then refinement steps until ||r|| / ||b|| < tol:
-//     inner Jacobi-PCG on A d = r / ||r|| in the inner precision (FP32
-//     values and vectors; dot products accumulated in FP64) to
-//     inner_tolerance, x += ||r|| d, r = b - A x in FP64.
-//   With the FP64 inner precision and max_outer = 1 this is plain FP64
-//   Jacobi-PCG with a true-residual check.
+//   r = b - A x (FP64); Jacobi-PCG in the inner precision (FP32 values and
+//   vectors, dot products accumulated in FP64) with reliable updates: the
+//   correction d accumulates in FP32 and, whenever the recurrence residual
+//   has dropped by inner_tolerance since the last update, x += d in FP64
+//   and the FP64 true residual replaces the recurrence one, keeping the
+//   search direction (no restart, so the iteration count stays that of
+//   plain PCG).  The solve ends when the FP64 true residual is below tol.
+//   With the FP64 inner precision this is plain FP64 Jacobi-PCG judged on
+//   the true residual.
 // Per inner iteration three passes over the system's rows, each ending in a
 // workgroup reduction (wavefront butterfly + LDS, fixed order):
 //   q = A p and <p,q>;  d += a p, r -= a q, <r, D^-1 r>, <r,r>;
@@ -199,20 +202,19 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
     if (bnorm == 0.0) {
       for (int i = tid; i < n; i += BLOCK) a.x[r0 + i] = 0.0;
       conv = 1;
+    } else if (rel < a.tol) {
+      conv = 1;
     }
-    while (!conv) {
-      rel = rnorm / bnorm;
-      if (rel < a.tol) {
-        conv = 1;
-        break;
-      }
-      if (outer >= a.max_outer) break;
-      ++outer;
-      const double inv = 1.0 / rnorm;
+    if (!conv) {
+      // Jacobi-PCG in the inner precision with reliable updates: the
+      // correction d accumulates in T; when |r| has dropped by inner_tol
+      // since the last update (or reaches the tolerance), x += d in FP64,
+      // r = b - A x in FP64 replaces the recurrence residual and the
+      // search direction is kept, so the Krylov recurrence is not restarted.
       double rz[1] = {0.0};
       for (int i = tid; i < n; i += BLOCK) {
         const int64_t g = r0 + i;
-        const T ri = (T)(a.r64[g] * inv);
+        const T ri = (T)a.r64[g];
         R[g] = ri;
         Dv[g] = (T)0;
         const T zi = invd[g] * ri;
@@ -221,6 +223,7 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
       }
       wg_sum<BLOCK, 1>(rz, red, phase);
       double rzv = rz[0];
+      double rupd = rnorm;  // |r| at the last FP64 update
       for (int it = 0; it < a.max_inner; ++it) {
         __syncthreads();  // p complete
         double pq[1] = {0.0};
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
           pq[0] += (double)pget(g) * (double)s;
         }
         wg_sum<BLOCK, 1>(pq, red, phase);
-        if (!(pq[0] > 0.0)) break;  // breakdown: leave it to the outer loop
+        if (!(pq[0] > 0.0)) break;  // breakdown
         const T alpha = (T)(rzv / pq[0]);
         double st[2] = {0.0, 0.0};
         for (int i = tid; i < n; i += BLOCK) {
@@ -246,7 +249,44 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
         }
         wg_sum<BLOCK, 2>(st, red, phase);
         ++inner_total;
-        if (sqrt(st[1]) < a.inner_tol) break;
+        const double rn = sqrt(st[1]);
+        if (rn < a.inner_tol * rupd || rn < a.tol * bnorm) {
+          // reliable update: x += d, true residual in FP64
+          for (int i = tid; i < n; i += BLOCK) {
+            const int64_t g = r0 + i;
+            a.x[g] += (double)Dv[g];
+            Dv[g] = (T)0;
+          }
+          __syncthreads();  // x complete
+          double rr[1] = {0.0};
+          for (int i = tid; i < n; i += BLOCK) {
+            const int64_t g = r0 + i;
+            double s = 0.0;
+            for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) s += a.v64[q] * a.x[a.ci[q]];
+            const double ri = a.b[g] - s;
+            a.r64[g] = ri;
+            rr[0] += ri * ri;
+          }
+          wg_sum<BLOCK, 1>(rr, red, phase);
+          rnorm = sqrt(rr[0]);
+          rel = rnorm / bnorm;
+          ++outer;
+          if (rel < a.tol) {
+            conv = 1;
+            break;
+          }
+          if (outer >= a.max_outer) break;
+          rupd = rnorm;
+          double rzn[1] = {0.0};
+          for (int i = tid; i < n; i += BLOCK) {
+            const int64_t g = r0 + i;
+            const T ri = (T)a.r64[g];
+            R[g] = ri;
+            rzn[0] += (double)ri * (double)(invd[g] * ri);
+          }
+          wg_sum<BLOCK, 1>(rzn, red, phase);
+          st[0] = rzn[0];
+        }
         const T beta = (T)(st[0] / rzv);
         rzv = st[0];
         for (int i = tid; i < n; i += BLOCK) {
@@ -254,22 +294,10 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
           pset(g, invd[g] * R[g] + beta * pget(g));
         }
       }
-      for (int i = tid; i < n; i += BLOCK) {
-        const int64_t g = r0 + i;
-        a.x[g] += rnorm * (double)Dv[g];
+      if (!conv) {  // budget spent: keep the correction made so far
+        for (int i = tid; i < n; i += BLOCK) a.x[r0 + i] += (double)Dv[r0 + i];
+        __syncthreads();
       }
-      __syncthreads();  // x complete
-      double rr[1] = {0.0};
-      for (int i = tid; i < n; i += BLOCK) {
-        const int64_t g = r0 + i;
-        double s = 0.0;
-        for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) s += a.v64[q] * a.x[a.ci[q]];
-        const double ri = a.b[g] - s;
-        a.r64[g] = ri;
-        rr[0] += ri * ri;
-      }
-      wg_sum<BLOCK, 1>(rr, red, phase);
-      rnorm = sqrt(rr[0]);
     }
     if (tid == 0) {
       a.iters[sid] = inner_total;
@@ -447,9 +475,11 @@ struct spk_batch_s {
     hipStream_t s = stream();
     const int prec = c.precision;
     BatchArgs a = args(c, b, x, prec, ord, K_);
-    a.lds_rows = t.lds_rows;
+    // the LDS bin threshold is capped to the budget of this precision
     const size_t es = prec == SPK_F32 ? 4 : 8;
-    const size_t dyn = (size_t)t.lds_rows * es;
+    const int64_t cap = 131072 / ((int64_t)es * std::max(1, t.wg_per_cu));
+    a.lds_rows = (int)std::min<int64_t>(t.lds_rows, cap);
+    const size_t dyn = (size_t)a.lds_rows * es;
     hipLaunchKernelGGL(batch_reset_kernel, dim3(nblk(std::max<int64_t>(reset_rows, 1))),
                        dim3(256), 0, s, reset_rows, x, counter.p);
     const int grid = std::max(1, t.wg_per_cu * ctx().num_cus);
@@ -670,7 +700,7 @@ spk_status spk_batch_pcg_solve(spk_batch B, const spk_batch_config* cfg,
   return guarded([&] {
     SPK_REQUIRE(B && d_b && d_x, SPK_INVALID_ARGUMENT, "null argument");
     check_config(cfg);
-    check_tuning(B->tuning, cfg->precision);
+    check_tuning(B->tuning, SPK_F32);  // lds_rows is capped per precision
     if (cfg->precision == SPK_F64) B->ensure_f64();  // no allocation in a capture
     hipStream_t s = stream();
     const spk_batch_tuning t = B->tuning;
@@ -758,8 +788,8 @@ spk_status spk_batch_autotune(spk_batch B, const spk_batch_config* cfg,
     SPK_HIP_TRY(hipMemcpyAsync(dord.p, ord.data(), m * 4, hipMemcpyHostToDevice, s));
     std::vector<spk_batch_tuning> cands;
     for (int bs : {256, 512, 1024})
-      for (int wg : {1, 2, 4, 8})
-        for (int lds : {0, 4096, 16384}) {
+      for (int wg : {1, 2, 4})
+        for (int lds : {0, 16384}) {
           spk_batch_tuning t{bs, wg, lds, 0};
           if (bs * wg > 2048) continue;
           const int64_t es = cfg->precision == SPK_F32 ? 4 : 8;
@@ -773,10 +803,10 @@ spk_status spk_batch_autotune(spk_batch B, const spk_batch_config* cfg,
     spk_batch_tuning bt = cands[0];
     int64_t k = 0;
     try {
+      // one untimed run, then one timed run per candidate (x is reset by
+      // the enqueued reset kernel over all rows)
+      B->enqueue(*cfg, d_b, d_x, cands[0], dord.p, (int)m, B->total);
       for (const auto& t : cands) {
-        // warm-up then one timed run over the sample (x is reset by the
-        // enqueued reset kernel over all rows)
-        B->enqueue(*cfg, d_b, d_x, t, dord.p, (int)m, B->total);
         SPK_HIP_TRY(hipEventRecord(e0, s));
         B->enqueue(*cfg, d_b, d_x, t, dord.p, (int)m, B->total);
         SPK_HIP_TRY(hipEventRecord(e1, s));

@@ -130,6 +130,9 @@ __device__ __forceinline__ void store_ready(T* p, T v) {
   typedef typename Sentinel<T>::U U;
   U u;
   __builtin_memcpy(&u, &v, sizeof(T));
+  // a NaN result carrying the sentinel's payload (propagated from a NaN
+  // input) must not read as "not ready": publish it with another payload
+  if (u == Sentinel<T>::bits) u ^= (U)1;
   __hip_atomic_store(reinterpret_cast<U*>(p), u, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -144,6 +147,10 @@ __device__ __forceinline__ T sentinel_value() {
 }
 
 constexpr unsigned kSpinLimit = 1u << 22;
+constexpr int kErrorWords = 64;  // sweep status word + guard words
+// sweep schedule thresholds (see spk_precond_s::level_launch)
+constexpr int64_t kLevelLaunchRows = 65536;
+constexpr int64_t kLevelLaunchMaxLevels = 4096;
 
 // LOWER: forward_solve (triangular_solve.hpp:42-53), the diagonal is the
 // row's last entry; otherwise backward_solve (:82-96), the diagonal is the
@@ -157,35 +164,65 @@ __global__ __launch_bounds__(256) void sweep_kernel(
     const int32_t* __restrict__ ci, const T* __restrict__ v, T* b, T* x,
     bool reset_b, T* mark, int* error) {
 #pragma clang fp contract(off)
+  // A small persistent grid walks the sweep order: each thread takes the
+  // rows t, t + W, t + 2W, ... (W = grid threads), so only a window of W
+  // rows -- a few levels -- waits at any time.  A wavefront always holds 64
+  // consecutive positions of one (padded) level, i.e. independent rows, and
+  // every row waits only for earlier positions, so the earliest unfinished
+  // row can always proceed.
+  const int W = gridDim.x * blockDim.x;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += W) {
+    const int i = rows[t];
+    if (i < 0) continue;  // level padding
+    const int p0 = rp[i], p1 = rp[i + 1];
+    const int q1 = LOWER ? p1 - 1 : p1;
+    const int pd = LOWER ? p1 - 1 : p0;
+    T s = b[i];
+    if (reset_b) b[i] = sentinel_value<T>();
+    if (mark) mark[i] = sentinel_value<T>();
+    int p = LOWER ? p0 : p0 + 1;
+    unsigned spins = 0;
+    bool done = false;
+    while (!done) {
+      if (p < q1) {
+        T xv;
+        if (load_ready(x + ci[p], &xv)) {
+          s = s - v[p] * xv;
+          ++p;
+        } else if (++spins > kSpinLimit) {
+          atomicOr(error, LOWER ? 1 : 2);
+          p = q1;
+        } else {
+          __builtin_amdgcn_s_sleep(2);  // back off: fewer polls in flight
+        }
+      }
+      if (p >= q1) {
+        store_ready(x + i, s / v[pd]);
+        done = true;
+      }
+    }
+    // keep this row's publication ahead of the next row's waits
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  }
+}
+
+// One dependency level of a triangular sweep: every row of the level reads
+// only rows of earlier levels (final, previous launches), so plain loads
+// suffice.  Same operation order as sweep_kernel.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void level_sweep_kernel(
+    const int32_t* __restrict__ rows, int count, const int32_t* __restrict__ rp,
+    const int32_t* __restrict__ ci, const T* __restrict__ v, const T* b, T* x) {
+#pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= count) return;
   const int i = rows[t];
-  if (i < 0) return;  // level padding
   const int p0 = rp[i], p1 = rp[i + 1];
   const int q1 = LOWER ? p1 - 1 : p1;
   const int pd = LOWER ? p1 - 1 : p0;
   T s = b[i];
-  if (reset_b) b[i] = sentinel_value<T>();
-  if (mark) mark[i] = sentinel_value<T>();
-  int p = LOWER ? p0 : p0 + 1;
-  unsigned spins = 0;
-  bool done = false;
-  while (!done) {
-    if (p < q1) {
-      T xv;
-      if (load_ready(x + ci[p], &xv)) {
-        s = s - v[p] * xv;
-        ++p;
-      } else if (++spins > kSpinLimit) {
-        atomicOr(error, 1);
-        p = q1;
-      }
-    }
-    if (p >= q1) {
-      store_ready(x + i, s / v[pd]);
-      done = true;
-    }
-  }
+  for (int p = LOWER ? p0 : p0 + 1; p < q1; ++p) s = s - v[p] * x[ci[p]];
+  x[i] = s / v[pd];
 }
 
 template <typename T>
@@ -279,23 +316,62 @@ float* spk_precond_s::tmp<float>() {
   return sentinel_buffer(tmp32, n);
 }
 
+void spk_precond_s::reset_sentinels() {
+  if (tmp64.p)
+    hipLaunchKernelGGL((fill_sentinel_kernel<double>), dim3(blocks_for(n)),
+                       dim3(256), 0, stream(), tmp64.p, n);
+  if (tmp32.p)
+    hipLaunchKernelGGL((fill_sentinel_kernel<float>), dim3(blocks_for(n)),
+                       dim3(256), 0, stream(), tmp32.p, n);
+  SPK_HIP_TRY(hipGetLastError());
+}
+
 void spk_precond_s::check_sweeps() {
   if (kind != kIlu) return;
-  int h = 0;
-  SPK_HIP_TRY(hipMemcpyAsync(&h, sweep_error.p, sizeof(int),
-                             hipMemcpyDeviceToHost, stream()));
+  // word 0: 1 = a forward, 2 = a backward sweep's bounded wait expired;
+  // words 1..kErrorWords-1 must stay zero (a guard against stray writes)
+  int h[kErrorWords];
+  SPK_HIP_TRY(hipMemcpyAsync(h, sweep_error.p, sizeof(h), hipMemcpyDeviceToHost,
+                             stream()));
   SPK_HIP_TRY(hipStreamSynchronize(stream()));
-  SPK_REQUIRE(!h, SPK_HIP_ERROR,
-              "ILU triangular sweep: bounded wait expired (results invalid)");
+  bool guard_ok = true;
+  for (int k = 1; k < kErrorWords; ++k) guard_ok = guard_ok && h[k] == 0;
+  if (h[0] == 0 && guard_ok) return;
+  SPK_HIP_TRY(hipMemsetAsync(sweep_error.p, 0, sizeof(h), stream()));
+  SPK_HIP_TRY(hipStreamSynchronize(stream()));
+  SPK_REQUIRE(guard_ok, SPK_HIP_ERROR,
+              "ILU sweep status words overwritten (a stray device write)");
+  SPK_REQUIRE(false, SPK_HIP_ERROR,
+              std::string("ILU triangular sweep: bounded wait expired in the ") +
+                  (h[0] & 1 ? "forward" : "") + (h[0] == 3 ? " and " : "") +
+                  (h[0] & 2 ? "backward" : "") + " sweep (results invalid)");
+}
+
+template <typename T, bool LOWER>
+void spk_precond_s::level_sweep(const T* b, T* x) {
+  hipStream_t s = stream();
+  CsrView<T> v = (LOWER ? L : U)->view<T>();
+  const std::vector<int32_t>& ptr = LOWER ? lvl_ptr_l : lvl_ptr_u;
+  const int32_t* rows = LOWER ? lvl_rows_l.p : lvl_rows_u.p;
+  for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+    const int cnt = ptr[l + 1] - ptr[l];
+    hipLaunchKernelGGL((level_sweep_kernel<T, LOWER>), dim3(blocks_for(cnt)),
+                       dim3(256), 0, s, rows + ptr[l], cnt, v.rp, v.ci, v.v, b, x);
+  }
+  SPK_HIP_TRY(hipGetLastError());
 }
 
 template <typename T>
 void spk_precond_s::forward(const T* b, T* x) {
+  if (level_launch) {
+    level_sweep<T, true>(b, x);
+    return;
+  }
   hipStream_t s = stream();
   CsrView<T> v = L->view<T>();
   hipLaunchKernelGGL((fill_sentinel_kernel<T>), dim3(blocks_for(n)),
                      dim3(256), 0, s, x, n);
-  hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(blocks_for(sweep_len_l)),
+  hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(sweep_grid_l),
                      dim3(256), 0, s, sweep_rows_l.p, (int)sweep_len_l, v.rp,
                      v.ci, v.v, (T*)b, x, false, (T*)nullptr, sweep_error.p);
   SPK_HIP_TRY(hipGetLastError());
@@ -303,11 +379,15 @@ void spk_precond_s::forward(const T* b, T* x) {
 
 template <typename T>
 void spk_precond_s::backward(const T* b, T* x) {
+  if (level_launch) {
+    level_sweep<T, false>(b, x);
+    return;
+  }
   hipStream_t s = stream();
   CsrView<T> v = U->view<T>();
   hipLaunchKernelGGL((fill_sentinel_kernel<T>), dim3(blocks_for(n)),
                      dim3(256), 0, s, x, n);
-  hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(blocks_for(sweep_len_u)),
+  hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(sweep_grid_u),
                      dim3(256), 0, s, sweep_rows_u.p, (int)sweep_len_u, v.rp,
                      v.ci, v.v, (T*)b, x, false, (T*)nullptr, sweep_error.p);
   SPK_HIP_TRY(hipGetLastError());
@@ -332,13 +412,18 @@ void spk_precond_s::apply(const T* in, T* out) {
       // reading in[i] (in may alias out), the backward sweep returns y to
       // the sentinel as it consumes it.
       T* y = tmp<T>();
+      if (level_launch) {  // wide levels: one launch per level, no waiting
+        level_sweep<T, true>(in, y);
+        level_sweep<T, false>((const T*)y, out);
+        break;
+      }
       hipStream_t s = stream();
       CsrView<T> lv = L->view<T>(), uv = U->view<T>();
-      hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(blocks_for(sweep_len_l)),
+      hipLaunchKernelGGL((sweep_kernel<T, true>), dim3(sweep_grid_l),
                          dim3(256), 0, s, sweep_rows_l.p, (int)sweep_len_l,
                          lv.rp, lv.ci, lv.v, (T*)in, y, false, out,
                          sweep_error.p);
-      hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(blocks_for(sweep_len_u)),
+      hipLaunchKernelGGL((sweep_kernel<T, false>), dim3(sweep_grid_u),
                          dim3(256), 0, s, sweep_rows_u.p, (int)sweep_len_u,
                          uv.rp, uv.ci, uv.v, y, out, true, (T*)nullptr,
                          sweep_error.p);
@@ -395,6 +480,35 @@ spk_status spk_precond_create_identity(int64_t n, spk_precond* out) {
     M->kind = spk_precond_s::kIdentity;
     M->n = n;
     *out = M;
+  });
+}
+
+spk_status spk_ilu0_set_schedule(spk_precond M, int schedule) {
+  return guarded([&] {
+    SPK_REQUIRE(M && M->kind == spk_precond_s::kIlu, SPK_INVALID_ARGUMENT,
+                "not an ILU(0) preconditioner");
+    SPK_REQUIRE(schedule >= 0 && schedule <= 2, SPK_INVALID_ARGUMENT,
+                "bad schedule");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    if (schedule == 0) {
+      const int64_t nl = (int64_t)M->lvl_ptr_l.size() - 1;
+      const int64_t nu = (int64_t)M->lvl_ptr_u.size() - 1;
+      M->level_launch = M->n >= kLevelLaunchRows &&
+                        std::max(nl, nu) <= kLevelLaunchMaxLevels;
+    } else {
+      M->level_launch = schedule == 1;
+    }
+    // the per-level schedule leaves ordinary values in the scratch vector;
+    // the sync-free one needs it filled with the ready sentinel
+    if (!M->level_launch) M->reset_sentinels();
+  });
+}
+
+spk_status spk_ilu0_get_schedule(spk_precond M, int* schedule) {
+  return guarded([&] {
+    SPK_REQUIRE(M && schedule && M->kind == spk_precond_s::kIlu,
+                SPK_INVALID_ARGUMENT, "not an ILU(0) preconditioner");
+    *schedule = M->level_launch ? 1 : 2;
   });
 }
 
@@ -464,8 +578,8 @@ spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out) {
     try {
       M->kind = spk_precond_s::kIlu;
       M->n = n;
-      M->sweep_error.alloc(1);
-      SPK_HIP_TRY(hipMemsetAsync(M->sweep_error.p, 0, sizeof(int), s));
+      M->sweep_error.alloc(kErrorWords);
+      SPK_HIP_TRY(hipMemsetAsync(M->sweep_error.p, 0, sizeof(int) * kErrorWords, s));
       std::vector<int32_t> rows_l, rows_u;
       M->lvl_ptr_l = group_levels(lvl_l, rows_l);
       M->lvl_ptr_u = group_levels(lvl_u, rows_u);
@@ -477,6 +591,22 @@ spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out) {
                   SPK_INVALID_ARGUMENT, "too many rows for the sweep order");
       M->sweep_len_l = (int64_t)pad_l.size();
       M->sweep_len_u = (int64_t)pad_u.size();
+      {
+        const int64_t nl = (int64_t)M->lvl_ptr_l.size() - 1;
+        const int64_t nu = (int64_t)M->lvl_ptr_u.size() - 1;
+        M->level_launch = n >= kLevelLaunchRows &&
+                          std::max(nl, nu) <= kLevelLaunchMaxLevels;
+        // sync-free window: about four padded levels of threads
+        auto grid = [&](int64_t len, int64_t levels) {
+          const int64_t w = levels > 0 ? len / levels : 64;
+          int64_t g = (4 * w + 255) / 256;
+          const int64_t cap = std::max<int64_t>(4, 4 * ctx().num_cus);
+          g = std::max<int64_t>(4, std::min(g, cap));
+          return (unsigned)std::min<int64_t>(g, blocks_for(std::max<int64_t>(len, 1)));
+        };
+        M->sweep_grid_l = grid(M->sweep_len_l, nl);
+        M->sweep_grid_u = grid(M->sweep_len_u, nu);
+      }
       M->sweep_rows_l.alloc(pad_l.size());
       M->sweep_rows_u.alloc(pad_u.size());
       SPK_HIP_TRY(hipMemcpyAsync(M->sweep_rows_l.p, pad_l.data(), pad_l.size() * 4,

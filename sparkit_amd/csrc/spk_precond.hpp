@@ -32,9 +32,16 @@ struct spk_precond_s {
   // the wavefront size (the sweeps' thread -> row map)
   spk::DevBuf<int32_t> sweep_rows_l, sweep_rows_u;
   int64_t sweep_len_l = 0, sweep_len_u = 0;
+  unsigned sweep_grid_l = 1, sweep_grid_u = 1;  // persistent sweep grids
   spk::DevBuf<double> tmp64;  // between the sweeps; kept filled with the
   spk::DevBuf<float> tmp32;   // ready sentinel between applications
   spk::DevBuf<int> sweep_error;  // set if a sweep's bounded wait expired
+  // Sweep schedule, chosen when the factors are made: one launch per
+  // dependency level when the levels are wide (average at least
+  // kLevelLaunchWidth rows: thousands of waiting wavefronts would flood the
+  // memory system with readiness polls), the sync-free sweep otherwise
+  // (long chains of narrow levels, where launches would dominate).
+  bool level_launch = false;
   // SPAI
   spk_csr_s* M = nullptr;  // not owned
 
@@ -51,8 +58,12 @@ struct spk_precond_s {
   void backward(const T* b, T* x);
   template <typename T>
   T* tmp();
+  template <typename T, bool LOWER>
+  void level_sweep(const T* b, T* x);
   // throws if a triangular sweep reported an expired wait (synchronises)
   void check_sweeps();
+  // refill the scratch vectors with the ready sentinel
+  void reset_sentinels();
 };
 
 template <>

@@ -15,6 +15,8 @@ enum : int {
   kWsBicgstab = 30,  // 30..44
   kWsIr = 45,        // 45..59
   kWsMisc = 60,
+  kWsCgs = 64,       // 64..74
+  kWsBicg = 75,      // 75..84
 };
 
 inline bool is_identity(const spk_precond_s* M) {

@@ -273,6 +273,26 @@ void test_errors() {
   }));
 }
 
+void test_cgs_bicg() {
+  // cgs_test.cpp:107-127 / bicg_test.cpp:139-173
+  auto Ah = tridiagonal(4, -1.0, 4.0, -1.0);
+  auto A = sh::Device_csr::from(Ah);
+  std::vector<double> xt{1, 2, 3, 4};
+  auto b = host_multiply(Ah, xt);
+  std::vector<double> x(4, 0.0);
+  sh::Cgs_config<double> cfg{1e-12, 100, false};
+  auto s = sh::cgs(b.begin(), b.end(), x.begin(), x.end(), cfg, A,
+                   sh::Device_identity{}, sh::Device_identity{});
+  CHECK(s.converged);
+  for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
+  std::fill(x.begin(), x.end(), 0.0);
+  auto At = sh::transpose(A);
+  auto s2 = sh::bicg(b.begin(), b.end(), x.begin(), x.end(), cfg, A, At,
+                     sh::jacobi(A), sh::Device_identity{});
+  CHECK(s2.converged);
+  for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
+}
+
 void test_learned_spai_and_batched() {
   // a 9-point anisotropic diffusion matrix from the device generator
   spk_csr g = nullptr;
@@ -320,6 +340,7 @@ int main() {
                {"gmres_fgmres_bicgstab", test_gmres_fgmres_bicgstab},
                {"device_vectors_and_fp32", test_device_vectors_and_fp32},
                {"errors", test_errors},
+               {"cgs_bicg", test_cgs_bicg},
                {"learned_spai_and_batched", test_learned_spai_and_batched}};
   for (const auto& c : cases) {
     int before = g_failures;

@@ -81,7 +81,7 @@ def test_graph_replay_and_tunings_agree(spk, batch):
         batch.solve(cfg, b, x1, use_graph=True)
         assert np.array_equal(x1.numpy(), ref), rep
         assert np.array_equal(batch.results()[0], it0)
-    for t in [(512, 1, 4096), (1024, 2, 0), (256, 4, 16384)]:
+    for t in [(512, 1, 4096), (1024, 2, 0), (256, 2, 16384)]:
         batch.tuning = t
         x2 = spk.DeviceVector.zeros(batch.total_rows)
         batch.solve(cfg, b, x2)

@@ -1,0 +1,70 @@
+"""Device transpose store and BiCG against the oracle (multiply_transpose and
+bicg.hpp restated, pinned in test_oracle_bicg.py)."""
+import numpy as np
+import pytest
+
+from test_oracle_cgs import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def dv(spk, a, dt=np.float64):
+    return spk.DeviceVector.from_numpy(np.asarray(a, dtype=dt))
+
+
+@pytest.mark.parametrize("which", ["kat", "rect", "convdiff", "powerlaw"])
+def test_transpose_spmv_bit_exact(O, spk, which):
+    if which == "kat":
+        A = O.make_matrix(3, 3, [0, 0, 1, 2, 2], [0, 2, 1, 0, 2], [2., 3., 4., 5., 6.])
+    elif which == "rect":
+        A = O.make_matrix(2, 3, [0, 0, 1], [0, 2, 1], [1., 2., 3.])
+    elif which == "convdiff":
+        A = O.convdiff_3d(9, 8, 7, 1.0, 0.3, 0.2, 0.1)
+    else:
+        A = O.powerlaw_spd(4000, 5)
+    At = spk.CsrMatrix.from_oracle(A).transpose()
+    assert (At.nrows, At.ncols, At.nnz) == (A.ncols, A.nrows, A.nnz)
+    x = O.uniform_pm1(3, A.nrows)
+    y = spk.multiply(At, dv(spk, x)).numpy()
+    yr = O.spmv_transpose(A, x)
+    if which == "powerlaw":  # long columns are summed by several lanes
+        absr = np.abs(A.dense().T) @ np.abs(x)
+        assert np.all(np.abs(y - yr) <= 1e-12 * absr)
+    else:
+        assert np.array_equal(y.view(np.uint64), yr.view(np.uint64))
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_reference_cases(O, spk, k):
+    name, A, b, xt, tol, margin = cases(O)[k]
+    mx = 200 if k >= 4 else 100
+    Ad = spk.CsrMatrix.from_oracle(A)
+    x = dv(spk, np.zeros(A.nrows))
+    s = spk.bicg(dv(spk, b), x, spk.BicgConfig(tol, mx, True), Ad, Ad.transpose())
+    xr, sr = O.bicg(A, b, tol=tol, max_iterations=mx)
+    assert s.converged and sr.converged, name
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 2, name
+    assert np.all(np.abs(x.numpy() - xt) <= margin), name
+
+
+@pytest.mark.parametrize("left,right", [("jacobi", None), (None, "jacobi"),
+                                        ("ilu", "jacobi")])
+def test_preconditioned_and_fp32(O, spk, left, right):
+    A = O.convdiff_centered_2d(12, 12, 1.0, 0.3, 0.3, 5.0)
+    b = O.spmv(A, np.arange(1.0, 145.0))
+    Ad = spk.CsrMatrix.from_oracle(A)
+    At = Ad.transpose()
+    mk = {"jacobi": spk.jacobi, "ilu": spk.ilu0, None: lambda _A: spk.identity}
+    x = dv(spk, np.zeros(A.nrows))
+    s = spk.bicg(dv(spk, b), x, spk.BicgConfig(1e-10, 600), Ad, At, mk[left](Ad),
+                 mk[right](Ad))
+    xr, sr = O.bicg(A, b, tol=1e-10, max_iterations=600, left=left, right=right)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 2
+    assert O.true_relres(A, b, x.numpy()) < 1e-9
+    x32 = dv(spk, np.zeros(A.nrows), np.float32)
+    s = spk.bicg(dv(spk, b, np.float32), x32,
+                 spk.BicgConfig(1e-5, 600, False, spk.F32), Ad, At, mk[left](Ad),
+                 mk[right](Ad))
+    assert s.converged
+    assert O.true_relres(A, b, x32.numpy().astype(np.float64)) < 1e-4

@@ -1,0 +1,68 @@
+"""Device CGS against the oracle's CGS (pinned by the reference's own
+cgs_test.cpp cases in test_oracle_cgs.py): same convergence, matvec counts
+within one step (tree-ordered dot products), solutions to the reference
+tests' margins, FP64 true residuals; FP32 vectors; both preconditioner
+slots."""
+import numpy as np
+import pytest
+
+from test_oracle_cgs import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def dv(spk, a, dt=np.float64):
+    return spk.DeviceVector.from_numpy(np.asarray(a, dtype=dt))
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_reference_cases(O, spk, k):
+    name, A, b, xt, tol, margin = cases(O)[k]
+    mx = 200 if k >= 4 else 100
+    Ad = spk.CsrMatrix.from_oracle(A)
+    x = dv(spk, np.zeros(A.nrows))
+    s = spk.cgs(dv(spk, b), x, spk.CgsConfig(tol, mx, True), Ad)
+    xr, sr = O.cgs(A, b, tol=tol, max_iterations=mx)
+    assert s.converged == sr.converged == True, name
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 2, name
+    assert np.all(np.abs(x.numpy() - xt) <= margin), name
+    assert len(s.iteration_residuals) == s.computed_iterations // 2
+
+
+def test_zero_rhs_budget_and_fp32(O, spk):
+    tri = O.tridiagonal_matrix(4, -1.0, 4.0, -1.0)
+    Ad = spk.CsrMatrix.from_oracle(tri)
+    x = dv(spk, np.ones(4))
+    s = spk.cgs(dv(spk, np.zeros(4)), x, spk.CgsConfig(1e-12, 100), Ad)
+    assert s.converged and np.all(x.numpy() == 0.0)
+    grid = O.poisson_2d(4, 4, 5.0)
+    b = O.spmv(grid, np.arange(1.0, 17.0))
+    Gd = spk.CsrMatrix.from_oracle(grid)
+    x = dv(spk, np.zeros(16))
+    s = spk.cgs(dv(spk, b), x, spk.CgsConfig(1e-14, 2), Gd)
+    assert not s.converged and s.computed_iterations <= 2
+    A = O.convdiff_centered_2d(10, 10, 1.0, 0.3, 0.3, 5.0)
+    b = O.spmv(A, np.arange(1.0, 101.0))
+    x = dv(spk, np.zeros(100), np.float32)
+    s = spk.cgs(dv(spk, b, np.float32), x,
+                spk.CgsConfig(1e-5, 400, False, spk.F32),
+                spk.CsrMatrix.from_oracle(A))
+    assert s.converged
+    assert O.true_relres(A, b, x.numpy().astype(np.float64)) < 1e-4
+
+
+@pytest.mark.parametrize("left,right", [("jacobi", None), (None, "jacobi"),
+                                        ("ilu", None), (None, "ilu"),
+                                        ("ilu", "jacobi")])
+def test_preconditioned(O, spk, left, right):
+    A = O.convdiff_centered_2d(12, 12, 1.0, 0.3, 0.3, 5.0)
+    b = O.spmv(A, np.arange(1.0, 145.0))
+    Ad = spk.CsrMatrix.from_oracle(A)
+    mk = {"jacobi": spk.jacobi, "ilu": spk.ilu0, None: lambda _A: spk.identity}
+    x = dv(spk, np.zeros(A.nrows))
+    s = spk.cgs(dv(spk, b), x, spk.CgsConfig(1e-10, 600), Ad, mk[left](Ad),
+                mk[right](Ad))
+    xr, sr = O.cgs(A, b, tol=1e-10, max_iterations=600, left=left, right=right)
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 2
+    assert O.true_relres(A, b, x.numpy()) < 1e-9

@@ -108,3 +108,45 @@ def test_ilu0_levels(O, spk):
     # natural-order 7-point stencil: i+j+k hyperplanes
     F = spk.ilu0(spk.CsrMatrix.from_oracle(O.convdiff_3d(6, 5, 4, 1., .1, .1, .1)))
     assert F.levels() == (6 + 5 + 4 - 2, 6 + 5 + 4 - 2)
+
+
+@pytest.mark.parametrize("schedule", ["levels", "sync-free"])
+def test_ilu0_schedules_bit_exact(O, spk, schedule):
+    """Both sweep schedules run the reference's arithmetic: applications,
+    standalone sweeps, repeated and in-place use are bit-identical to the
+    oracle."""
+    for A in (O.convdiff_3d(13, 11, 9, 1.0, 0.3, 0.2, 0.1),
+              O.convdiff_centered_2d(17, 15, 1.0, 0.3, 0.2, 3.0)):
+        Lr, Ur = O.ilu0(A)
+        F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+        F.schedule = schedule
+        assert F.schedule == schedule
+        for seed in (1, 2):  # repeated applications
+            r = O.uniform_pm1(seed, A.nrows)
+            z = spk.DeviceVector.zeros(A.nrows)
+            spk.ilu_apply(F, dv(spk, r), z)
+            z_ref = O.backward_solve(Ur, O.forward_solve(Lr, r))
+            assert np.array_equal(z.numpy(), z_ref)
+        zz = dv(spk, r)              # in place
+        F.apply(zz, zz)
+        assert np.array_equal(zz.numpy(), z_ref)
+        y = spk.DeviceVector.zeros(A.nrows)
+        F.forward(dv(spk, r), y)
+        assert np.array_equal(y.numpy(), O.forward_solve(Lr, r))
+        w = spk.DeviceVector.zeros(A.nrows)
+        F.backward(y, w)
+        assert np.array_equal(w.numpy(), z_ref)
+        # switching schedules on a used preconditioner keeps it exact
+        F.schedule = "sync-free" if schedule == "levels" else "levels"
+        z2 = spk.DeviceVector.zeros(A.nrows)
+        F.apply(dv(spk, r), z2)
+        assert np.array_equal(z2.numpy(), z_ref)
+        F.schedule = "auto"
+
+
+def test_ilu0_auto_schedule_picks_levels_for_wide_levels(O, spk):
+    A = spk.convdiff_3d(48, 48, 48, 1.0, 0.1, 0.05, 0.02)
+    F = spk.ilu0(A)
+    assert F.schedule == "levels"          # 110592 rows, 142 levels
+    S = spk.ilu0(spk.CsrMatrix.from_oracle(O.tridiagonal_matrix(300, -1., 4., -1.)))
+    assert S.schedule == "sync-free"
