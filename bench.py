@@ -40,6 +40,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM-side bytes per SpMV measured with rocprofv3 PMC counters (FETCH_SIZE x
+# 2.0 + WRITE_SIZE, calibrated on a 256 MiB copy) for the default
+# configurations; the commands and the raw counts are in
+# profiles/spmv_traffic_pmc.md
+MEASURED_TRAFFIC = {"3D 7-point Poisson 192^3": 800.6e6,
+                    "power-law SPD n=4194304": 107.2e9}
 METRIC = ("time-to-1e-8 residual: learned-FGMRES vs Jacobi-PCG; "
           "SpMV GB/s % HBM peak")
 
@@ -537,7 +543,9 @@ def main():
                      "achieved": dom["spmv_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": round(dom["spmv_GBs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": MEASURED_TRAFFIC.get(dom["input"]),
+                     "traffic_source": "profiles/spmv_traffic_pmc.md"
+                     if dom["input"] in MEASURED_TRAFFIC else None},
         "copy_GBs": round(copy_gbs, 1),
         "wall_s_timed": round(wall, 3),
     }
