@@ -174,7 +174,23 @@ def mixed_precision_section(spk, grid=128, seed=42):
            "ilu0_apply_ms": sched,
            "nnz": A.nnz, "peclet": round(pe, 3), "ilu0_setup_ms": round(t_ilu, 2),
            "target_true_relres": 1e-10}
-    for oname, ortho in (("cgs2", spk.ORTHO_CGS2), ("mgs", spk.ORTHO_MGS)):
+    runs = [(0, "cgs2", spk.ORTHO_CGS2), (0, "mgs", spk.ORTHO_MGS)]
+    runs += [(k, "cgs2", spk.ORTHO_CGS2) for k in (2, 3, 4)]
+    timer = spk.Timer()
+    for k, oname, ortho in runs:
+        F.jacobi_sweeps = k
+        tag = "ilu0" if k == 0 else f"ilu0jac{k}"
+        if k:
+            app = {}
+            for dt in (np.float64, np.float32):
+                r = spk.DeviceVector.zeros(n, dt)
+                z = spk.DeviceVector.zeros(n, dt)
+                F.apply(r, z)
+                timer.start()
+                for _ in range(3):
+                    F.apply(r, z)
+                app[np.dtype(dt).name] = round(timer.stop() / 3, 3)
+            res[f"ilu0_jacobi{k}_apply_ms"] = app
         x64 = spk.DeviceVector.zeros(n)
         cfg = spk.GmresConfig(1e-10, 50, 5000, False, spk.F64, ortho)
         spk.gmres(b, spk.DeviceVector.zeros(n),
@@ -182,7 +198,7 @@ def mixed_precision_section(spk, grid=128, seed=42):
                   spk.identity, F)  # warm-up
         s64, t64 = _timed(lambda: spk.gmres(b, x64, cfg, A, spk.identity, F))
         r64 = spk.true_relres(A, b, x64)
-        res[f"fp64_gmres50_ilu0_{oname}"] = {
+        res[f"fp64_gmres50_{tag}_{oname}"] = {
             "iterations": s64.computed_iterations, "converged": s64.converged,
             "ms": round(t64, 2), "true_relres": r64}
         xir = spk.DeviceVector.zeros(n)
@@ -195,17 +211,26 @@ def mixed_precision_section(spk, grid=128, seed=42):
         sir, tir = _timed(lambda: spk.ir_solve(b, xir, icfg, A, F))
         rir = spk.true_relres(A, b, xir)
         a64, air = x64.numpy(), xir.numpy()
-        res[f"ir_fp32_gmres50_ilu0_{oname}"] = {
+        res[f"ir_fp32_gmres50_{tag}_{oname}"] = {
             "outer_iterations": sir.outer_iterations,
             "inner_iterations": sir.inner_iterations,
             "converged": sir.converged, "ms": round(tir, 2), "true_relres": rir,
             "rel_error_vs_fp64_solve": float(np.linalg.norm(air - a64) /
                                              np.linalg.norm(a64))}
-        res[f"speedup_ir_vs_fp64_{oname}"] = round(t64 / tir, 3)
-        log(f"mixed precision ({oname}): FP64 GMRES {s64.computed_iterations}"
-            f" it {t64:.1f} ms relres {r64:.2e}; IR {sir.outer_iterations} "
-            f"outer {sir.inner_iterations} inner {tir:.1f} ms relres "
-            f"{rir:.2e}")
+        res[f"speedup_ir_vs_fp64_{tag}_{oname}"] = round(t64 / tir, 3)
+        log(f"mixed precision ({tag}, {oname}): FP64 GMRES "
+            f"{s64.computed_iterations} it {t64:.1f} ms relres {r64:.2e}; IR "
+            f"{sir.outer_iterations} outer {sir.inner_iterations} inner "
+            f"{tir:.1f} ms relres {rir:.2e}")
+    F.jacobi_sweeps = 0
+    # against the reference's method (FP64 GMRES(50), exact ILU(0) apply)
+    base = min(res["fp64_gmres50_ilu0_cgs2"]["ms"],
+               res["fp64_gmres50_ilu0_mgs"]["ms"])
+    best = min((v["ms"], k) for k, v in res.items()
+               if isinstance(v, dict) and v.get("converged")
+               and v.get("true_relres", 1.0) < 1e-10 and "ms" in v)
+    res["fastest_to_1e-10"] = {"solve": best[1], "ms": best[0],
+                               "speedup_vs_fp64_exact_ilu0": round(base / best[0], 3)}
     return res
 
 
@@ -271,7 +296,7 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     res["ilu0_apply_ms"] = ilu_schedules(spk, F, n)
     res["ilu0_schedule"] = F.schedule
     res["ilu0_levels"] = F.levels()
-    cgi = spk.CGConfig(1e-8, 1000, 1000)
+    cgi = spk.CGConfig(1e-8, 200, 200)  # diverges: a bounded sample suffices
     run("ilu0_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cgi, A, F),
         warm=False)
     run("fgmres30_learned_fp64", lambda x: spk.fgmres(b, x, fg, A, Ml))

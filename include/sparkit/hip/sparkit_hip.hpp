@@ -231,6 +231,16 @@ class Device_ilu0 : public Device_preconditioner {
     check(spk_precond_create_ilu0(A.handle(), modified ? 1 : 0, &p), "ilu0");
     adopt(p, A.rows());
   }
+  // 0: exact triangular solves (ilu_apply); k > 0: k Jacobi sweeps per
+  // triangle (an approximate, bandwidth-bound apply)
+  void set_jacobi_sweeps(int k) {
+    check(spk_ilu0_set_jacobi_sweeps(h_.get(), k), "spk_ilu0_set_jacobi_sweeps");
+  }
+  int jacobi_sweeps() const {
+    int k = 0;
+    check(spk_ilu0_get_jacobi_sweeps(h_.get(), &k), "spk_ilu0_get_jacobi_sweeps");
+    return k;
+  }
 };
 
 // z = M r with an explicit sparse approximate inverse

@@ -191,6 +191,19 @@ spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out);
 #define SPK_ILU_SYNC_FREE 2
 spk_status spk_ilu0_set_schedule(spk_precond M, int schedule);
 spk_status spk_ilu0_get_schedule(spk_precond M, int* schedule);
+/* Approximate ILU(0) apply (no reference counterpart; the reference's
+ * ilu_apply, ilu.hpp:173-179, solves both triangles exactly, which stays the
+ * default, sweeps = 0).  With sweeps = k > 0 each triangular solve becomes k
+ * Jacobi iterations started from the diagonal-scaled right-hand side:
+ *   y_0 = r,       y_j = r - L_s y_{j-1}
+ *   z_0 = y_k / d, z_j = (y_k - U_s z_{j-1}) / d      (d: U's diagonal)
+ * with L_s, U_s the strict triangles -- a fixed linear operator (k + 1
+ * terms of each factor's Neumann series, equal to the exact solve once k
+ * reaches the dependency depth), made of 2k + 1 bandwidth-bound passes
+ * instead of two latency-bound sweeps.  spk_ilu0_forward/backward stay
+ * exact.  k in [0, 1024]. */
+spk_status spk_ilu0_set_jacobi_sweeps(spk_precond M, int sweeps);
+spk_status spk_ilu0_get_jacobi_sweeps(spk_precond M, int* sweeps);
 /* Identity slot (conjugate_gradient_test.cpp:45-47). */
 spk_status spk_precond_create_identity(int64_t n, spk_precond* out);
 /* z = M r for an explicit sparse approximate inverse M (a store). */

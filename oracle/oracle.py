@@ -55,6 +55,11 @@ class IluD(C.Structure):
     _fields_ = [("L", CsrD), ("U", CsrD), ("tmp", C.c_void_p)]
 
 
+class IluJacD(C.Structure):
+    _fields_ = [("L", CsrD), ("U", CsrD), ("sweeps", C.c_int64),
+                ("tmp", C.c_void_p)]
+
+
 class ConfigD(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("restart", C.c_int64),
                 ("max_iterations", C.c_int64)]
@@ -103,6 +108,8 @@ def _declare(L):
         for side in ("forward", "backward"):
             getattr(L, f"orc_{side}_solve_{nm}").argtypes = [
                 C.c_int64, _i64p, _i64p, fp, fp, fp]
+        getattr(L, f"orc_apply_ilu_jacobi_{nm}").argtypes = [
+            C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     L.orc_true_relres_d.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _f64p,
                                     _f64p]
     L.orc_true_relres_d.restype = C.c_double
@@ -323,6 +330,28 @@ def ilu0(A: Csr, modified: bool = False):
             Csr(n, n, urp, uci[:nu].copy(), uv[:nu].copy()))
 
 
+def _ilu_jac_struct(Lf, Uf, k, dtype):
+    lv = np.ascontiguousarray(Lf.vals, dtype)
+    uv = np.ascontiguousarray(Uf.vals, dtype)
+    tmp = np.zeros(3 * Lf.nrows, dtype)
+    s = IluJacD(Lf._struct(lv), Uf._struct(uv), int(k), _ptr(tmp))
+    return s, [Lf, Uf, lv, uv, tmp, s]
+
+
+def ilu_jacobi_apply(Lf: Csr, Uf: Csr, k: int, r):
+    """Approximate ilu_apply by k Jacobi sweeps per triangle (no reference
+    counterpart; replaces the exact solves of ilu.hpp:173-179)."""
+    r = np.ascontiguousarray(r)
+    dtype = r.dtype
+    sfx = "f" if dtype == np.float32 else "d"
+    s, keep = _ilu_jac_struct(Lf, Uf, k, dtype)
+    out = np.zeros_like(r)
+    getattr(lib(), f"orc_apply_ilu_jacobi_{sfx}")(
+        C.cast(C.pointer(s), C.c_void_p), len(r), _ptr(r), _ptr(out))
+    del keep
+    return out
+
+
 def forward_solve(L: Csr, b):
     """triangular_solve.hpp:27-55."""
     b = np.ascontiguousarray(b)
@@ -404,6 +433,17 @@ class _Op:
             s = IluD(Lf._struct(lv), Uf._struct(uv), _ptr(tmp))
             self.keep += [Lf, Uf, lv, uv, tmp, s]
             self.fn = C.cast(getattr(L, f"orc_apply_ilu_{sfx}"), C.c_void_p)
+            self.ctx = C.cast(C.pointer(s), C.c_void_p)
+        elif kind == "ilu_jac":
+            # ("ilu_jac", k) or ("ilu_jac", (L, U, k))
+            if isinstance(data, tuple):
+                Lf, Uf, k = data
+            else:
+                (Lf, Uf), k = ilu0(A), data
+            s, keep = _ilu_jac_struct(Lf, Uf, k, dtype)
+            self.keep += keep
+            self.fn = C.cast(getattr(L, f"orc_apply_ilu_jacobi_{sfx}"),
+                             C.c_void_p)
             self.ctx = C.cast(C.pointer(s), C.c_void_p)
         else:
             raise ValueError(f"unknown operator kind {kind!r}")

@@ -306,6 +306,48 @@ ORC_TRISOLVE(f, float)
 ORC_OPS(d, double, orc_csr_d, orc_ilu_d)
 ORC_OPS(f, float, orc_csr_f, orc_ilu_f)
 
+/* Approximate ilu_apply (no reference counterpart; it replaces the exact
+ * triangular solves of ilu.hpp:173-179 / triangular_solve.hpp:27-98 by k
+ * Jacobi iterations each):
+ *   y_0 = r, y_j = r - L_s y_{j-1};  z_0 = y_k / d, z_j = (y_k - U_s z_{j-1}) / d
+ * L_s: every L row but its last entry (the unit diagonal); U_s: every U
+ * row but its first (the diagonal d).  Row sums in storage order from 0. */
+#define ORC_ILU_JAC(SFX, T, JAC)                                               \
+  void orc_apply_ilu_jacobi_##SFX(const void* ctx, int64_t n, const T* in,     \
+                                  T* out) {                                    \
+    const JAC* F = (const JAC*)ctx;                                            \
+    const int64_t k = F->sweeps;                                               \
+    T* y = F->tmp;                                                             \
+    T* w = F->tmp + n;                                                         \
+    T* z = F->tmp + 2 * n;                                                     \
+    const int64_t *lrp = F->L.row_ptr, *lci = F->L.col_ind;                    \
+    const int64_t *urp = F->U.row_ptr, *uci = F->U.col_ind;                    \
+    memcpy(y, in, (size_t)n * sizeof(T));                                      \
+    for (int64_t j = 0; j < k; ++j) {                                          \
+      for (int64_t i = 0; i < n; ++i) {                                        \
+        T t = (T)0;                                                            \
+        for (int64_t p = lrp[i]; p < lrp[i + 1] - 1; ++p)                      \
+          t += F->L.vals[p] * y[lci[p]];                                       \
+        w[i] = in[i] - t;                                                      \
+      }                                                                        \
+      memcpy(y, w, (size_t)n * sizeof(T));                                     \
+    }                                                                          \
+    for (int64_t i = 0; i < n; ++i) z[i] = y[i] / F->U.vals[urp[i]];           \
+    for (int64_t j = 0; j < k; ++j) {                                          \
+      for (int64_t i = 0; i < n; ++i) {                                        \
+        T t = (T)0;                                                            \
+        for (int64_t p = urp[i] + 1; p < urp[i + 1]; ++p)                      \
+          t += F->U.vals[p] * z[uci[p]];                                       \
+        w[i] = (y[i] - t) / F->U.vals[urp[i]];                                 \
+      }                                                                        \
+      memcpy(z, w, (size_t)n * sizeof(T));                                     \
+    }                                                                          \
+    memcpy(out, z, (size_t)n * sizeof(T));                                     \
+  }
+
+ORC_ILU_JAC(d, double, orc_ilu_jacobi_d)
+ORC_ILU_JAC(f, float, orc_ilu_jacobi_f)
+
 /* multiply_transpose(A, x): sparse_blas.hpp:36-52 -- y = 0, then for each row
  * i in order and each entry (i, j) in storage order y_j += a_ij x_i */
 #define ORC_SPMV_T(SFX, T, CSR)                                                \

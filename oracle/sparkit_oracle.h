@@ -160,6 +160,19 @@ typedef struct {
   float* tmp;
 } orc_ilu_f;
 
+/* approximate ILU apply: factors, Jacobi sweeps per triangle, 3n scratch */
+typedef struct {
+  orc_csr_d L, U;
+  int64_t sweeps;
+  double* tmp;
+} orc_ilu_jacobi_d;
+
+typedef struct {
+  orc_csr_f L, U;
+  int64_t sweeps;
+  float* tmp;
+} orc_ilu_jacobi_f;
+
 /* identity slot: conjugate_gradient_test.cpp:45-47 */
 void orc_apply_identity_d(const void* ctx, int64_t n, const double* in, double* out);
 void orc_apply_identity_f(const void* ctx, int64_t n, const float* in, float* out);
@@ -183,6 +196,12 @@ void orc_apply_jacobi_f(const void* ctx, int64_t n, const float* in, float* out)
 /* ilu_apply lambda: ilu.hpp:173-179, ilu_test.cpp:127-130 */
 void orc_apply_ilu_d(const void* ctx, int64_t n, const double* in, double* out);
 void orc_apply_ilu_f(const void* ctx, int64_t n, const float* in, float* out);
+/* approximate ilu_apply by Jacobi sweeps (no reference counterpart; ctx
+ * points at an orc_ilu_jacobi_*) */
+void orc_apply_ilu_jacobi_d(const void* ctx, int64_t n, const double* in,
+                            double* out);
+void orc_apply_ilu_jacobi_f(const void* ctx, int64_t n, const float* in,
+                            float* out);
 
 /* ------------------------------------------------------------- solvers */
 

@@ -331,6 +331,18 @@ class Ilu0(Preconditioner):
         call("spk_ilu0_set_schedule", self.h,
              {"auto": 0, "levels": 1, "sync-free": 2}[name])
 
+    @property
+    def jacobi_sweeps(self) -> int:
+        """0: the exact triangular solves of ilu_apply; k > 0: k Jacobi
+        sweeps per triangle (approximate apply, no reference counterpart)."""
+        v = C.c_int()
+        call("spk_ilu0_get_jacobi_sweeps", self.h, C.byref(v))
+        return v.value
+
+    @jacobi_sweeps.setter
+    def jacobi_sweeps(self, k: int):
+        call("spk_ilu0_set_jacobi_sweeps", self.h, int(k))
+
     def forward(self, b: DeviceVector, x: DeviceVector):
         call("spk_ilu0_forward", self.h, _code(b.dtype), b.ptr, x.ptr)
 

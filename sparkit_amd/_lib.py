@@ -144,6 +144,8 @@ _SIGS = {
     "spk_ilu0_levels": ([vp, P_i64, P_i64], None),
     "spk_ilu0_set_schedule": ([vp, i32], None),
     "spk_ilu0_get_schedule": ([vp, C.POINTER(C.c_int)], None),
+    "spk_ilu0_set_jacobi_sweeps": ([vp, i32], None),
+    "spk_ilu0_get_jacobi_sweeps": ([vp, C.POINTER(C.c_int)], None),
     "spk_ilu0_forward": ([vp, i32, vp, vp], None),
     "spk_ilu0_backward": ([vp, i32, vp, vp], None),
     "spk_cg_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,

@@ -12,11 +12,15 @@
 // polls the flag once per chunk.  With M_L = Jacobi and M_R = identity
 // (Jacobi-PCG) one iteration is three kernels plus two finishers:
 //   K1  q = A p, partial <p,q>;  finisher: alpha = rz/<p,q>
-//   K2  x += alpha p; r -= alpha q; <r,r>, <r, D^-1 r>;  finisher: |r|,
-//       convergence flag, beta, rz
-//   K3  p = D^-1 r + beta p
-// HBM traffic per iteration: B_spmv + 11 s n bytes (K2 reads x,p,r,q,D^-1
-// and writes x,r; K3 reads r,D^-1,p and writes p).
+//   K2  r -= alpha q; <r,r>, <r, D^-1 r>;  finisher: |r|, convergence flag,
+//       beta, rz
+//   K3  x += alpha p; p = D^-1 r + beta p
+// K3 is the one pass that reads the old p, so the x update rides there.  It
+// must also run in the step whose K2 raised the convergence flag, so it is
+// gated on the step count (it runs iff K2 of the same step ran) instead of
+// the flag.
+// HBM traffic per iteration: B_spmv + 10 s n bytes (K2 reads r,q,D^-1 and
+// writes r; K3 reads x,r,D^-1,p and writes x,p).
 #include "spk_solver_common.hpp"
 
 namespace spk {
@@ -152,7 +156,32 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     // compute_step_size(), :232-239
     if (!right_id) apply_precond(MR, n, (const T*)p, hatp);
     launch_spmv(*A, (const T*)hatp, EpiCgDot<T>{p, q, S, done}, s);
-    if (jac) {
+    if (jac && right_id) {
+      // compute_new_residual + is_converged + the z, rz_new of
+      // update_direction (:248-292), fused
+      launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {
+        const T a = (T)S->alpha;
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)ri * (double)(invd[i] * ri);
+      }, [=] __device__(const double* t) {
+        cg_check(S, t[0]);
+        if (!S->done) {
+          S->beta = t[1] / S->rz;
+          S->rz = t[1];
+        }
+      }, done);
+      // compute_new_solution (:241-246) + the direction update (:278-292);
+      // after the last step the new direction is computed and unused
+      const int64_t step = k + 1;
+      launch_map(n, [=] __device__(int64_t i) {
+        if (S->iters != step) return;
+        const T pi = p[i];
+        x[i] = (T)S->alpha * pi + x[i];
+        p[i] = invd[i] * r[i] + (T)S->beta * pi;
+      });
+    } else if (jac) {
       // compute_new_solution/residual + is_converged + the z, rz_new of
       // update_direction (:241-292), fused
       launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {

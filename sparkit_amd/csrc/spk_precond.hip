@@ -282,6 +282,8 @@ using namespace spk;
 spk_precond_s::~spk_precond_s() {
   delete L;
   delete U;
+  delete Ls;
+  delete Us;
 }
 
 const float* spk_precond_s::inv_diag32() {
@@ -393,6 +395,130 @@ void spk_precond_s::backward(const T* b, T* x) {
   SPK_HIP_TRY(hipGetLastError());
 }
 
+// strict part of one factor row: L drops its last entry (the unit
+// diagonal), U its first (the diagonal, kept in d)
+__global__ void strict_copy_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                   const int32_t* __restrict__ ci,
+                                   const double* __restrict__ v, bool lower,
+                                   int32_t* __restrict__ sci,
+                                   double* __restrict__ sv,
+                                   double* __restrict__ d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = rp[i], b = rp[i + 1];
+  if (lower) {
+    for (int k = a; k < b - 1; ++k) { sci[k - i] = ci[k]; sv[k - i] = v[k]; }
+  } else {
+    d[i] = v[a];
+    for (int k = a + 1; k < b; ++k) { sci[k - i - 1] = ci[k]; sv[k - i - 1] = v[k]; }
+  }
+}
+
+void spk_precond_s::build_strict() {
+  if (Ls) return;
+  hipStream_t s = stream();
+  du64.alloc((size_t)(n > 0 ? n : 1));
+  for (int f = 0; f < 2; ++f) {
+    spk_csr_s* F = f == 0 ? L : U;
+    std::vector<int32_t> hrp((size_t)n + 1);
+    for (int64_t i = 0; i <= n; ++i) hrp[(size_t)i] = F->h_rp[(size_t)i] - (int32_t)i;
+    auto* S = new spk_csr_s();
+    try {
+      S->allocate(n, n, F->nnz - n);
+      SPK_HIP_TRY(hipMemcpyAsync(S->rp.p, hrp.data(), (n + 1) * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, s));
+      if (n > 0)
+        hipLaunchKernelGGL(strict_copy_kernel, dim3(blocks_for(n)), dim3(256), 0,
+                           s, n, (const int32_t*)F->rp.p, (const int32_t*)F->ci.p,
+                           (const double*)F->v64.p, f == 0, S->ci.p, S->v64.p,
+                           du64.p);
+      SPK_HIP_TRY(hipGetLastError());
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      finish_store(*S, false);
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    (f == 0 ? Ls : Us) = S;
+  }
+}
+
+template <typename T>
+struct EpiJacBackward {  // z = (y - U_s z_old) / d
+  static constexpr bool kReduce = false;
+  const T* __restrict__ y;
+  const T* __restrict__ d;
+  T* __restrict__ z;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const {
+    z[i] = (y[i] - s) / d[i];
+    return 0.0;
+  }
+  __device__ void finalize(double) const {}
+};
+
+template <typename T>
+static T* scratch_vec(DevBuf<T>& b, int64_t n) {
+  if (b.n < (size_t)n) b.alloc((size_t)(n > 0 ? n : 1));
+  return b.p;
+}
+
+template <typename T>
+static void jac_buffers(spk_precond_s& P, T** a, T** b);
+template <>
+void jac_buffers<double>(spk_precond_s& P, double** a, double** b) {
+  *a = scratch_vec(P.jb64a, P.n);
+  *b = scratch_vec(P.jb64b, P.n);
+}
+template <>
+void jac_buffers<float>(spk_precond_s& P, float** a, float** b) {
+  *a = scratch_vec(P.jb32a, P.n);
+  *b = scratch_vec(P.jb32b, P.n);
+}
+
+template <typename T>
+static const T* diag_u(spk_precond_s& P);
+template <>
+const double* diag_u<double>(spk_precond_s& P) { return P.du64.p; }
+template <>
+const float* diag_u<float>(spk_precond_s& P) {
+  if (!P.du_has32) {
+    P.du32.alloc((size_t)(P.n > 0 ? P.n : 1));
+    if (P.n > 0)
+      hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(P.n)), dim3(256), 0,
+                         stream(), (const double*)P.du64.p, P.du32.p, P.n);
+    SPK_HIP_TRY(hipGetLastError());
+    P.du_has32 = true;
+  }
+  return P.du32.p;
+}
+
+template <typename T>
+void spk_precond_s::jacobi_apply(const T* in, T* out) {
+  hipStream_t s = stream();
+  const int k = jac_sweeps;
+  T *a, *b;
+  jac_buffers<T>(*this, &a, &b);
+  const T* d = diag_u<T>(*this);
+  // forward: y_0 = r, y_j = r - L_s y_{j-1}; y_k ends in a
+  const T* y = in;
+  for (int j = 0; j < k; ++j) {
+    T* dst = (k - j) % 2 == 1 ? a : b;
+    launch_spmv(*Ls, y, EpiResidual<T>{in, dst}, s);
+    y = dst;
+  }
+  // backward: z_0 = y / d, z_j = (y - U_s z_{j-1}) / d; z_k ends in out
+  // (in may alias out: it is no longer read)
+  T* z0 = k % 2 == 0 ? out : b;
+  launch_map(n, [=] __device__(int64_t i) { z0[i] = y[i] / d[i]; });
+  const T* z = z0;
+  for (int j = 0; j < k; ++j) {
+    T* dst = (k - j) % 2 == 1 ? out : b;
+    launch_spmv(*Us, z, EpiJacBackward<T>{y, d, dst}, s);
+    z = dst;
+  }
+}
+
 template <typename T>
 void spk_precond_s::apply(const T* in, T* out) {
   switch (kind) {
@@ -411,6 +537,10 @@ void spk_precond_s::apply(const T* in, T* out) {
       // Two launches: the forward sweep marks `out` with the sentinel after
       // reading in[i] (in may alias out), the backward sweep returns y to
       // the sentinel as it consumes it.
+      if (jac_sweeps > 0) {
+        jacobi_apply(in, out);
+        break;
+      }
       T* y = tmp<T>();
       if (level_launch) {  // wide levels: one launch per level, no waiting
         level_sweep<T, true>(in, y);
@@ -501,6 +631,26 @@ spk_status spk_ilu0_set_schedule(spk_precond M, int schedule) {
     // the per-level schedule leaves ordinary values in the scratch vector;
     // the sync-free one needs it filled with the ready sentinel
     if (!M->level_launch) M->reset_sentinels();
+  });
+}
+
+spk_status spk_ilu0_set_jacobi_sweeps(spk_precond M, int sweeps) {
+  return guarded([&] {
+    SPK_REQUIRE(M && M->kind == spk_precond_s::kIlu, SPK_INVALID_ARGUMENT,
+                "not an ILU(0) preconditioner");
+    SPK_REQUIRE(sweeps >= 0 && sweeps <= 1024, SPK_INVALID_ARGUMENT,
+                "sweeps must be in [0, 1024]");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    if (sweeps > 0) M->build_strict();
+    M->jac_sweeps = sweeps;
+  });
+}
+
+spk_status spk_ilu0_get_jacobi_sweeps(spk_precond M, int* sweeps) {
+  return guarded([&] {
+    SPK_REQUIRE(M && sweeps && M->kind == spk_precond_s::kIlu,
+                SPK_INVALID_ARGUMENT, "not an ILU(0) preconditioner");
+    *sweeps = M->jac_sweeps;
   });
 }
 

@@ -42,6 +42,18 @@ struct spk_precond_s {
   // memory system with readiness polls), the sync-free sweep otherwise
   // (long chains of narrow levels, where launches would dominate).
   bool level_launch = false;
+  // Approximate apply (jac_sweeps > 0): each triangular solve is replaced
+  // by jac_sweeps Jacobi iterations from the diagonal-scaled right-hand
+  // side, y <- r - L_s y and z <- (y - U_s z) / d_U, with L_s, U_s the
+  // strict parts (a fixed linear operator: k terms of the Neumann series;
+  // exact once k reaches the dependency depth, the strict parts being
+  // nilpotent).  The strict stores and d_U are made on first use.
+  int jac_sweeps = 0;
+  spk_csr_s* Ls = nullptr;
+  spk_csr_s* Us = nullptr;
+  spk::DevBuf<double> du64, jb64a, jb64b;
+  spk::DevBuf<float> du32, jb32a, jb32b;
+  bool du_has32 = false;
   // SPAI
   spk_csr_s* M = nullptr;  // not owned
 
@@ -64,6 +76,10 @@ struct spk_precond_s {
   void check_sweeps();
   // refill the scratch vectors with the ready sentinel
   void reset_sentinels();
+  // the strict parts and U's diagonal for the approximate apply
+  void build_strict();
+  template <typename T>
+  void jacobi_apply(const T* in, T* out);
 };
 
 template <>

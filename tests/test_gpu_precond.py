@@ -150,3 +150,58 @@ def test_ilu0_auto_schedule_picks_levels_for_wide_levels(O, spk):
     assert F.schedule == "levels"          # 110592 rows, 142 levels
     S = spk.ilu0(spk.CsrMatrix.from_oracle(O.tridiagonal_matrix(300, -1., 4., -1.)))
     assert S.schedule == "sync-free"
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 7])
+def test_ilu0_jacobi_sweeps_match_oracle(O, spk, k):
+    """Approximate apply (k Jacobi sweeps per triangle; no reference
+    counterpart): on stencil rows the device sums each row in storage order
+    and divides like the oracle, so FP64 results are bit-identical; FP32
+    within FP32 rounding; in-place use and switching back to exact solves."""
+    for A in (O.convdiff_3d(13, 11, 9, 1.0, 0.3, 0.2, 0.1),
+              O.convdiff_centered_2d(17, 15, 1.0, 0.3, 0.2, 3.0)):
+        Lr, Ur = O.ilu0(A)
+        F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+        F.jacobi_sweeps = k
+        assert F.jacobi_sweeps == k
+        for seed in (1, 2):
+            r = O.uniform_pm1(seed, A.nrows)
+            z = spk.DeviceVector.zeros(A.nrows)
+            F.apply(dv(spk, r), z)
+            z_ref = O.ilu_jacobi_apply(Lr, Ur, k, r)
+            assert np.array_equal(z.numpy(), z_ref)
+        zz = dv(spk, r)
+        F.apply(zz, zz)
+        assert np.array_equal(zz.numpy(), z_ref)
+        z32 = spk.DeviceVector.zeros(A.nrows, np.float32)
+        F.apply(dv(spk, r.astype(np.float32)), z32)
+        ref32 = O.ilu_jacobi_apply(Lr, Ur, k, r.astype(np.float32))
+        assert np.allclose(z32.numpy(), ref32, rtol=0,
+                           atol=1e-5 * np.abs(ref32).max())
+        # standalone sweeps stay exact; sweeps = 0 restores ilu_apply
+        y = spk.DeviceVector.zeros(A.nrows)
+        F.forward(dv(spk, r), y)
+        assert np.array_equal(y.numpy(), O.forward_solve(Lr, r))
+        F.jacobi_sweeps = 0
+        z0 = spk.DeviceVector.zeros(A.nrows)
+        F.apply(dv(spk, r), z0)
+        assert np.array_equal(z0.numpy(),
+                              O.backward_solve(Ur, O.forward_solve(Lr, r)))
+
+
+def test_ilu0_jacobi_sweeps_reach_exact_at_depth(O, spk):
+    """The strict triangles are nilpotent: with as many sweeps as dependency
+    levels the approximate apply is the exact ilu_apply."""
+    A = O.convdiff_3d(7, 6, 5, 1.0, 0.3, 0.2, 0.1)
+    Lr, Ur = O.ilu0(A)
+    F = spk.ilu0(spk.CsrMatrix.from_oracle(A))
+    F.jacobi_sweeps = max(F.levels())
+    r = O.uniform_pm1(3, A.nrows)
+    z = spk.DeviceVector.zeros(A.nrows)
+    F.apply(dv(spk, r), z)
+    ex = O.backward_solve(Ur, O.forward_solve(Lr, r))
+    assert np.allclose(z.numpy(), ex, rtol=0, atol=1e-13 * np.abs(ex).max())
+    with pytest.raises(spk.InvalidArgument):
+        F.jacobi_sweeps = -1
+    with pytest.raises(spk.InvalidArgument):  # not an ILU preconditioner
+        spk.Ilu0.jacobi_sweeps.fset(spk.jacobi(spk.CsrMatrix.from_oracle(A)), 2)

@@ -266,3 +266,58 @@ def test_ir_with_cgs2(O, spk):
     cfg = spk.IrConfig(1e-10, 20, 1e-4, 40, 400, spk.IR_GMRES, spk.ORTHO_CGS2)
     s = spk.ir_solve(spk.DeviceVector.from_numpy(b), x, cfg, Ad, spk.ilu0(Ad))
     assert s.converged and O.true_relres(A, b, x.numpy()) < 1e-10
+
+
+@pytest.mark.parametrize("max_it", [1, 4, 5, 6, 11, 23])
+@pytest.mark.parametrize("panels", [1, 3])
+def test_cg_jacobi_iterate_after_max_iterations(O, spk, max_it, panels):
+    """Jacobi-PCG applies the x update in the direction-update pass, which
+    is gated on the step count (it must also run in the step that stops
+    the loop): a solve stopped at any step, with restarts every 5 steps,
+    returns the oracle's iterate (conjugate_gradient.hpp:208-292)."""
+    A = O.poisson_2d(20, 17, 0.05)
+    b = O.uniform_pm1(11, A.nrows)
+    x0 = O.uniform_pm1(12, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    Ad.set_panels(panels)
+    bd, xd = dv(spk, b), dv(spk, x0)
+    cfg = spk.CGConfig(1e-30, 5, max_it, True, spk.F64)
+    s = spk.conjugate_gradient(bd, xd, cfg, Ad, spk.jacobi(Ad), spk.identity)
+    xr, sr = O.cg(A, b, x0, tol=1e-30, restart_iterations=5,
+                  max_iterations=max_it, left="jacobi", right=None)
+    assert not s.converged and s.computed_iterations == max_it
+    x = xd.numpy()
+    assert np.allclose(x, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+    assert np.allclose(s.iteration_residuals, sr.iteration_residuals,
+                       rtol=1e-9)
+    # FP32 vectors: same recurrence, FP32 roundings
+    xf = dv(spk, x0, np.float32)
+    cfg32 = spk.CGConfig(1e-30, 5, max_it, False, spk.F32)
+    s32 = spk.conjugate_gradient(dv(spk, b, np.float32), xf, cfg32, Ad,
+                                 spk.jacobi(Ad), spk.identity)
+    assert s32.computed_iterations == max_it
+    assert np.allclose(xf.numpy(), xr, rtol=0, atol=1e-4 * np.abs(xr).max())
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_gmres_with_approximate_ilu(O, spk, k):
+    """GMRES(50) with the Jacobi-sweep ILU(0) apply (a fixed linear
+    preconditioner) follows the oracle's GMRES with the same operator."""
+    A = O.convdiff_3d(14, 12, 10, 1.0, 0.4, 0.2, 0.1)
+    b = O.uniform_pm1(9, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    F = spk.ilu0(Ad)
+    F.jacobi_sweeps = k
+    x = spk.DeviceVector.zeros(A.nrows)
+    cfg = spk.GmresConfig(1e-10, 50, 500, True, spk.F64)
+    s = spk.gmres(spk.DeviceVector.from_numpy(b), x, cfg, Ad, F, spk.identity)
+    xr, sr = O.gmres(A, b, tol=1e-10, restart_dimension=50, max_iterations=500,
+                     left=("ilu_jac", k))
+    assert s.converged and sr.converged
+    assert abs(s.computed_iterations - sr.computed_iterations) <= 1
+    assert O.true_relres(A, b, x.numpy()) < 1e-9
+    # FP32-inner refinement with the same preconditioner
+    x2 = spk.DeviceVector.zeros(A.nrows)
+    icfg = spk.IrConfig(1e-10, 20, 1e-4, 50, 500, spk.IR_GMRES, spk.ORTHO_CGS2)
+    s2 = spk.ir_solve(spk.DeviceVector.from_numpy(b), x2, icfg, Ad, F)
+    assert s2.converged and O.true_relres(A, b, x2.numpy()) < 1e-10
