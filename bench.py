@@ -291,6 +291,17 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     fg = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_CGS2)
     fgm = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_MGS)
     run("jacobi_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cg, A, J))
+    # the learned inverse's symmetric part (SPD here) in the CG slot, FP64
+    # and as FP32 PCG with FP64 reliable updates
+    Msym, t_sym = _timed(lambda: model.predict(A).symmetric_part())
+    Ms = spk.spai(Msym)
+    res["symmetric_part_ms"] = round(t_sym, 2)
+    run("learned_sym_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cg, A, Ms))
+    rcfg = spk.IrConfig(tolerance=1e-8, max_outer=200, inner_tolerance=1e-2,
+                        restart=0, max_inner=50000,
+                        inner_method=spk.IR_CG_RELIABLE)
+    run("jacobi_relcg_fp32", lambda x: spk.ir_solve(b, x, rcfg, A, J))
+    run("learned_sym_relcg_fp32", lambda x: spk.ir_solve(b, x, rcfg, A, Ms))
     F, t_ilu = _timed(lambda: spk.ilu0(A))
     res["ilu0_setup_ms"] = round(t_ilu, 2)
     res["ilu0_apply_ms"] = ilu_schedules(spk, F, n)
@@ -308,6 +319,8 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                         orthogonalization=spk.ORTHO_CGS2)
     run("ir_fp32_fgmres30_learned", lambda x: spk.ir_solve(b, x, icfg, A, Ml))
     jt = res["jacobi_pcg_fp64"]["ms"]
+    for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32"):
+        res[f"speedup_{k}_vs_jacobi_pcg"] = round(jt / res[k]["ms"], 3)
     res["speedup_learned_ir_vs_jacobi_pcg"] = round(
         jt / res["ir_fp32_fgmres30_learned"]["ms"], 3)
     res["speedup_learned_vs_untrained_fgmres"] = round(

@@ -123,6 +123,12 @@ spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
  * multiply_transpose(A, x) (sparse_blas.hpp:36-52) with each output summed
  * in the reference's order. */
 spk_status spk_csr_transpose(spk_csr A, spk_csr* out);
+/* (M + M^T) / 2 on M's own pattern (no reference counterpart: the SPD part
+ * of a nonsymmetric approximate inverse, usable in conjugate_gradient's
+ * preconditioner slot).  The pattern must be structurally symmetric
+ * (SPK_INVALID_ARGUMENT otherwise); entry (i,j) = 0.5 (m_ij + m_ji), so the
+ * result is exactly symmetric. */
+spk_status spk_csr_symmetric_part(spk_csr M, spk_csr* out);
 
 /* Permutations on the device, perm (device, int64, perm[old] = new) of
  * nrows (rows, both) or ncols (columns) entries: rperm (mode
@@ -307,8 +313,18 @@ spk_status spk_bicg_solve(spk_csr A, spk_csr At, spk_precond M_left,
  *   (FP32 vectors and SpMV, FP64 dot products, Arnoldi/Givens scalars) to
  *   inner_tolerance;  x += ||r|| d in FP64.
  * inner_method: SPK_IR_GMRES (M as right preconditioner), SPK_IR_FGMRES,
- * SPK_IR_CG (M as left preconditioner), SPK_IR_BICGSTAB (M right). */
-enum { SPK_IR_GMRES = 0, SPK_IR_FGMRES = 1, SPK_IR_CG = 2, SPK_IR_BICGSTAB = 3 };
+ * SPK_IR_CG (M as left preconditioner), SPK_IR_BICGSTAB (M right), or
+ * SPK_IR_CG_RELIABLE: one FP32 preconditioned CG recurrence (FP64 dots)
+ * that is not restarted per step; instead, whenever its residual has
+ * dropped by inner_tolerance since the last update (or below tolerance),
+ * x += the FP32 correction in FP64, the residual is replaced by the FP64
+ * true residual and the search direction is kept (reliable updates, van
+ * der Vorst and Ye).  For it max_outer bounds the updates, max_inner the
+ * total FP32 steps, and restart is unused; a Jacobi, identity or explicit
+ * approximate-inverse (spk_precond_create_spmv) M keeps every kernel of a
+ * step device-gated, other preconditioners are applied ungated. */
+enum { SPK_IR_GMRES = 0, SPK_IR_FGMRES = 1, SPK_IR_CG = 2, SPK_IR_BICGSTAB = 3,
+       SPK_IR_CG_RELIABLE = 4 };
 
 typedef struct {
   double tolerance;       /* FP64 true relative residual target */

@@ -648,6 +648,22 @@ def aniso_q1_2d(N, seed, sigma=1.5, isotropic=False):
     return Csr(n, n, rp, ci, v)
 
 
+def symmetric_part(M: Csr) -> Csr:
+    """(M + M^T) / 2 on M's pattern, which must be structurally symmetric
+    (no reference counterpart)."""
+    n = M.nrows
+    pos = {}
+    for i in range(n):
+        for k in range(M.row_ptr[i], M.row_ptr[i + 1]):
+            pos[(i, int(M.col_ind[k]))] = k
+    v = np.empty(M.nnz)
+    for (i, j), k in pos.items():
+        if (j, i) not in pos:
+            raise OracleError(2, "pattern is not structurally symmetric")
+        v[k] = 0.5 * (M.vals[k] + M.vals[pos[(j, i)]])
+    return Csr(n, M.ncols, M.row_ptr.copy(), M.col_ind.copy(), v)
+
+
 def is_valid_permutation(perm):
     return bool(lib().orc_is_valid_permutation(
         len(perm), np.ascontiguousarray(perm, np.int64)))

@@ -40,7 +40,7 @@ __all__ = [
     "true_relres",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
-    "IR_BICGSTAB",
+    "IR_BICGSTAB", "IR_CG_RELIABLE",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
     "BatchedSystems",
     "poisson_2d", "poisson_3d", "convdiff_3d", "powerlaw_spd", "aniso_q1_2d",
@@ -198,6 +198,12 @@ class CsrMatrix:
         """A^T on the device; its SpMV is multiply_transpose(A, x)."""
         h = C.c_void_p()
         call("spk_csr_transpose", self.h, C.byref(h))
+        return CsrMatrix(h)
+
+    def symmetric_part(self) -> "CsrMatrix":
+        """(M + M^T) / 2 on M's (structurally symmetric) pattern."""
+        h = C.c_void_p()
+        call("spk_csr_symmetric_part", self.h, C.byref(h))
         return CsrMatrix(h)
 
     def permute(self, perm, mode="both") -> "CsrMatrix":
@@ -690,7 +696,7 @@ def bicgstab(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
                                        C.byref(c), b.ptr, x.ptr), cfg)
 
 
-IR_GMRES, IR_FGMRES, IR_CG, IR_BICGSTAB = 0, 1, 2, 3
+IR_GMRES, IR_FGMRES, IR_CG, IR_BICGSTAB, IR_CG_RELIABLE = 0, 1, 2, 3, 4
 
 
 @dataclass

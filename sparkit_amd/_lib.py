@@ -161,6 +161,7 @@ _SIGS = {
     "spk_bicg_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                         C.POINTER(Summary), vp], None),
     "spk_csr_transpose": ([vp, C.POINTER(vp)], None),
+    "spk_csr_symmetric_part": ([vp, C.POINTER(vp)], None),
     "spk_true_relres": ([vp, vp, vp, P_dbl], None),
     "spk_ir_solve": ([vp, vp, C.POINTER(IrConfig), vp, vp,
                       C.POINTER(IrSummary), vp], None),

@@ -168,10 +168,73 @@ spk_csr_s* transpose(spk_csr_s& A) {
   return assemble(A.ncols, A.nrows, A.nnz, c.p, r.p, A.v64.p);
 }
 
+// (M + M^T) / 2 on M's own pattern: one thread per row finds the mirror
+// entry of each of its entries by binary search in the mirror row
+__global__ void sym_part_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                const int32_t* __restrict__ ci,
+                                const double* __restrict__ v,
+                                double* __restrict__ out, int* err) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = rp[i]; k < rp[i + 1]; ++k) {
+    const int j = ci[k];
+    int lo = rp[j], hi = rp[j + 1];
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if (ci[mid] < (int)i) lo = mid + 1; else hi = mid;
+    }
+    if (lo < rp[j + 1] && ci[lo] == (int)i) {
+      out[k] = 0.5 * (v[k] + v[lo]);
+    } else {
+      out[k] = 0.0;
+      atomicOr(err, 1);
+    }
+  }
+}
+
+spk_csr_s* symmetric_part(spk_csr_s& M) {
+  SPK_REQUIRE(M.nrows == M.ncols, SPK_INVALID_ARGUMENT,
+              "the symmetric part needs a square matrix");
+  hipStream_t s = stream();
+  auto* S = new spk_csr_s();
+  try {
+    S->allocate(M.nrows, M.ncols, M.nnz);
+    SPK_HIP_TRY(hipMemcpyAsync(S->rp.p, M.rp.p, (M.nrows + 1) * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, s));
+    if (M.nnz > 0)
+      SPK_HIP_TRY(hipMemcpyAsync(S->ci.p, M.ci.p, M.nnz * sizeof(int32_t),
+                                 hipMemcpyDeviceToDevice, s));
+    DevBuf<int> err(1);
+    SPK_HIP_TRY(hipMemsetAsync(err.p, 0, 4, s));
+    if (M.nrows > 0)
+      hipLaunchKernelGGL(sym_part_kernel, dim3(nb(M.nrows)), dim3(256), 0, s,
+                         M.nrows, (const int32_t*)M.rp.p, (const int32_t*)M.ci.p,
+                         (const double*)M.v64.p, S->v64.p, err.p);
+    SPK_HIP_TRY(hipGetLastError());
+    int h = 0;
+    SPK_HIP_TRY(hipMemcpyAsync(&h, err.p, 4, hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    SPK_REQUIRE(!h, SPK_INVALID_ARGUMENT,
+                "pattern is not structurally symmetric");
+    finish_store(*S, false);
+  } catch (...) {
+    delete S;
+    throw;
+  }
+  return S;
+}
+
 }  // namespace
 }  // namespace spk
 
 using namespace spk;
+
+extern "C" spk_status spk_csr_symmetric_part(spk_csr M, spk_csr* out) {
+  return guarded([&] {
+    SPK_REQUIRE(M && out, SPK_INVALID_ARGUMENT, "null argument");
+    *out = symmetric_part(*M);
+  });
+}
 
 extern "C" spk_status spk_csr_transpose(spk_csr A, spk_csr* out) {
   return guarded([&] {

@@ -33,6 +33,224 @@ struct EpiResidualNorm {  // r = b - A x, partial |r|^2
   __device__ void finalize(double t) const { st->rnorm = sqrt(t); }
 };
 
+// ---------------------------------------------------------------------
+// SPK_IR_CG_RELIABLE: one FP32 preconditioned CG recurrence whose residual
+// is replaced by the FP64 true residual at "reliable updates" (van der
+// Vorst and Ye's scheme), instead of a fresh inner CG per refinement step.
+//   x64: FP64 solution; xi: FP32 correction since the last update
+//   FP32 step: q = A p; alpha = rz / <p,q>; xi += alpha p; r -= alpha q;
+//              z = M r; beta = <r,z> / rz; p = z + beta p
+//   update when |r| <= delta |r_last update| (delta = inner_tolerance) or
+//   |r| / |b| < tolerance:  x64 += xi; xi = 0; r64 = b - A x64 (FP64);
+//   r = (float) r64; z = M r; beta = <r,z> / rz; p = z + beta p -- the
+//   search direction is kept, so the Krylov progress survives the update.
+// Convergence is decided on the FP64 true residual at updates only.  The
+// host queues FP32 steps in chunks; a device flag (hold) turns the queued
+// steps into no-ops once an update is due, and the host then issues it.
+
+struct RcgState {
+  double bnorm, rnorm, rupd, rz, alpha, beta, tol, delta;
+  int64_t iters, updates;
+  int32_t hold, converged, pad0, pad1;
+};
+
+struct EpiRcgDot {  // q = A p, partial <p, q>
+  static constexpr bool kReduce = true;
+  const float* __restrict__ p;
+  float* __restrict__ q;
+  RcgState* st;
+  const int* done;
+  __device__ double row(int i, float s) const {
+    q[i] = s;
+    return (double)p[i] * (double)s;
+  }
+  __device__ void finalize(double pq) const { st->alpha = st->rz / pq; }
+};
+
+struct EpiRcgPrec {  // z = M r (an explicit approximate inverse), <r, z>
+  static constexpr bool kReduce = true;
+  const float* __restrict__ r;
+  float* __restrict__ z;
+  RcgState* st;
+  const int* done;
+  __device__ double row(int i, float s) const {
+    z[i] = s;
+    return (double)r[i] * (double)s;
+  }
+  __device__ void finalize(double rz) const {
+    st->beta = rz / st->rz;
+    st->rz = rz;
+  }
+};
+
+struct EpiRcgTrue {  // r64 = b - A x64, |r64|^2
+  static constexpr bool kReduce = true;
+  const double* __restrict__ b;
+  double* __restrict__ r;
+  RcgState* st;
+  const int* done = nullptr;
+  __device__ double row(int i, double s) const {
+    double ri = b[i] - s;
+    r[i] = ri;
+    return ri * ri;
+  }
+  __device__ void finalize(double t) const { st->rnorm = sqrt(t); }
+};
+
+// after the FP32 residual norm of a step: is an update due?
+__device__ inline bool rcg_due(RcgState* st, double rr) {
+  const double rn = sqrt(rr);
+  st->iters += 1;
+  if (rn <= st->delta * st->rupd || rn / st->bnorm < st->tol) {
+    st->hold = 1;
+    return true;
+  }
+  return false;
+}
+
+void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
+                 const double* b, double* x, double* r, spk_ir_summary* out,
+                 double* history, double bnorm, double rnorm) {
+  const int64_t n = A->nrows;
+  Workspace& ws = workspace();
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  float* r32 = (float*)ws.get(kWsIr + 1, nn * 4);
+  float* xi = (float*)ws.get(kWsIr + 2, nn * 4);
+  float* z = (float*)ws.get(kWsIr + 4, nn * 4);
+  float* p = (float*)ws.get(kWsIr + 5, nn * 4);
+  float* q = (float*)ws.get(kWsIr + 6, nn * 4);
+  DevState<RcgState> st(kWsIr + 7);
+  RcgState h{};
+  h.bnorm = bnorm;
+  h.rnorm = rnorm;
+  h.rupd = rnorm;
+  h.tol = cfg->tolerance;
+  h.delta = cfg->inner_tolerance;
+  h.rz = 1.0;  // the first "beta" below multiplies p = 0
+  st.upload(h);
+  RcgState* S = st.d;
+  const int* hold = &S->hold;
+  hipStream_t s = stream();
+  const bool jac = is_jacobi(M), ident = is_identity(M);
+  const bool spai = M && M->kind == spk_precond_s::kSpmv;
+  const float* invd = jac ? M->inv_diag<float>() : nullptr;
+
+  // r = (float) r64, xi = 0, z = M r, beta = <r,z> / rz, rz = <r,z>; then
+  // p = z + beta p (not when the update converged)
+  auto restart = [&](bool first) {
+    if (first) launch_map(n, [=] __device__(int64_t i) { p[i] = 0.0f; });
+    if (jac || ident) {
+      launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+        const float ri = (float)r[i];
+        const float zi = invd ? invd[i] * ri : ri;
+        r32[i] = ri;
+        z[i] = zi;
+        xi[i] = 0.0f;
+        acc[0] += (double)ri * (double)zi;
+      }, [=] __device__(const double* t) {
+        S->beta = t[0] / S->rz;
+        S->rz = t[0];
+      });
+    } else {
+      launch_map(n, [=] __device__(int64_t i) {
+        r32[i] = (float)r[i];
+        xi[i] = 0.0f;
+      });
+      if (spai) {
+        launch_spmv(*M->M, (const float*)r32, EpiRcgPrec{r32, z, S, nullptr}, s);
+      } else {
+        M->apply((const float*)r32, z);
+        launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+          acc[0] += (double)r32[i] * (double)z[i];
+        }, [=] __device__(const double* t) {
+          S->beta = t[0] / S->rz;
+          S->rz = t[0];
+        });
+      }
+    }
+    launch_map(n, [=] __device__(int64_t i) {
+      p[i] = z[i] + (float)S->beta * p[i];
+    }, hold);
+  };
+
+  auto step = [&]() {
+    launch_spmv(*A, (const float*)p, EpiRcgDot{p, q, S, hold}, s);
+    if (jac || ident) {
+      launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {
+        const float a = (float)S->alpha;
+        xi[i] = a * p[i] + xi[i];
+        const float ri = r32[i] - a * q[i];
+        r32[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)ri * (double)(invd ? invd[i] * ri : ri);
+      }, [=] __device__(const double* t) {
+        if (!rcg_due(S, t[0])) {
+          S->beta = t[1] / S->rz;
+          S->rz = t[1];
+        }
+      }, hold);
+      launch_map(n, [=] __device__(int64_t i) {
+        p[i] = (invd ? invd[i] * r32[i] : r32[i]) + (float)S->beta * p[i];
+      }, hold);
+    } else {
+      launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+        const float a = (float)S->alpha;
+        xi[i] = a * p[i] + xi[i];
+        const float ri = r32[i] - a * q[i];
+        r32[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+      }, [=] __device__(const double* t) { rcg_due(S, t[0]); }, hold);
+      if (spai) {
+        launch_spmv(*M->M, (const float*)r32, EpiRcgPrec{r32, z, S, hold}, s);
+      } else {  // a general preconditioner apply is not gated
+        M->apply((const float*)r32, z);
+        launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+          acc[0] += (double)r32[i] * (double)z[i];
+        }, [=] __device__(const double* t) {
+          S->beta = t[0] / S->rz;
+          S->rz = t[0];
+        }, hold);
+      }
+      launch_map(n, [=] __device__(int64_t i) {
+        p[i] = z[i] + (float)S->beta * p[i];
+      }, hold);
+    }
+  };
+
+  restart(true);
+  int64_t issued = 0, k = 0;
+  if (history) history[0] = rnorm / bnorm;
+  for (;;) {
+    // FP32 steps until an update is due or the budget is spent
+    int64_t chunk = 4;
+    while (issued < cfg->max_inner) {
+      const int64_t e = std::min(cfg->max_inner, issued + chunk);
+      for (; issued < e; ++issued) step();
+      if (st.download().hold) break;
+      chunk = std::min<int64_t>(chunk * 2, 64);
+    }
+    // reliable update: x64 += xi, FP64 true residual, convergence test
+    launch_map(n, [=] __device__(int64_t i) { x[i] += (double)xi[i]; });
+    launch_spmv(*A, (const double*)x, EpiRcgTrue{b, r, S}, s);
+    launch_finish<1>(reduce_partials(1), 0, [=] __device__(const double*) {
+      S->updates += 1;
+      S->rupd = S->rnorm;
+      S->converged = S->rnorm / S->bnorm < S->tol ? 1 : 0;
+      S->hold = S->converged;
+    }, nullptr);
+    h = st.download();
+    ++k;
+    if (history) history[k] = h.rnorm / h.bnorm;
+    if (h.converged || k >= cfg->max_outer || issued >= cfg->max_inner) break;
+    restart(false);
+  }
+  out->relres = h.rnorm / h.bnorm;
+  out->outer_iterations = h.updates;
+  out->inner_iterations = h.iters;
+  out->converged = h.converged;
+  out->status = SPK_OK;
+}
+
 void ir_solve(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
               const double* b, double* x, spk_ir_summary* out,
               double* history) {
@@ -56,6 +274,16 @@ void ir_solve(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
     launch_map(n, [=] __device__(int64_t i) { x[i] = 0.0; });
     out->converged = 1;
     SPK_HIP_TRY(hipStreamSynchronize(s));
+    return;
+  }
+  if (cfg->inner_method == SPK_IR_CG_RELIABLE) {
+    if (h.rnorm / h.bnorm < cfg->tolerance) {
+      if (history) history[0] = h.rnorm / h.bnorm;
+      out->relres = h.rnorm / h.bnorm;
+      out->converged = 1;
+      return;
+    }
+    cg_reliable(A, M, cfg, b, x, r, out, history, h.bnorm, h.rnorm);
     return;
   }
   spk_solver_config inner{};
@@ -127,6 +355,9 @@ extern "C" spk_status spk_ir_solve(spk_csr A, spk_precond M,
                 "preconditioner size mismatch");
     SPK_REQUIRE(cfg->max_outer >= 0 && cfg->max_inner >= 0, SPK_INVALID_ARGUMENT,
                 "negative iteration budget");
+    SPK_REQUIRE(cfg->inner_method >= SPK_IR_GMRES &&
+                    cfg->inner_method <= SPK_IR_CG_RELIABLE,
+                SPK_INVALID_ARGUMENT, "unknown inner method");
     ir_solve(A, M, cfg, d_b, d_x, out, history);
     check_preconds(M, nullptr);
   });

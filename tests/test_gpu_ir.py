@@ -76,3 +76,63 @@ def test_ir_zero_rhs_and_exact_start(O, spk):
     x = spk.DeviceVector.from_numpy(xt)
     s = spk.ir_solve(spk.DeviceVector.from_numpy(b), x, spk.IrConfig(), Ad)
     assert s.converged and s.outer_iterations == 0
+
+
+def _relcg(spk, Ad, b, M, tol=1e-10, delta=1e-2, max_outer=50, max_inner=5000):
+    x = spk.DeviceVector.zeros(Ad.nrows)
+    cfg = spk.IrConfig(tolerance=tol, max_outer=max_outer, inner_tolerance=delta,
+                       restart=0, max_inner=max_inner,
+                       inner_method=spk.IR_CG_RELIABLE)
+    s = spk.ir_solve(spk.DeviceVector.from_numpy(b), x, cfg, Ad, M)
+    return s, x.numpy()
+
+
+@pytest.mark.parametrize("prec", ["jacobi", "identity", "spai", "ilu"])
+def test_cg_reliable_updates(O, spk, prec):
+    """FP32 PCG with FP64 reliable updates reaches the FP64 true residual
+    that FP32 alone cannot, in about the FP64 PCG's iteration count (the
+    search direction survives each update), and agrees with the oracle's
+    FP64 CG solution."""
+    A = O.poisson_3d(22, 20, 18)
+    b = O.uniform_pm1(5, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    left = {"jacobi": "jacobi", "identity": None, "spai": "jacobi",
+            "ilu": "ilu"}[prec]
+    M = {"jacobi": lambda: spk.jacobi(Ad), "identity": lambda: None,
+         # an untrained network predicts the Jacobi inverse exactly
+         "spai": lambda: spk.spai(spk.SpaiModel(1).predict(Ad).symmetric_part()),
+         "ilu": lambda: spk.ilu0(Ad)}[prec]()
+    s, x = _relcg(spk, Ad, b, M, tol=1e-11)
+    assert s.converged and s.outer_iterations >= 2
+    rel = O.true_relres(A, b, x)
+    assert rel < 1e-11 and abs(rel - s.relres) <= 1e-3 * rel
+    h = np.array(s.relres_history)
+    assert len(h) == s.outer_iterations + 1 and np.all(h[1:] < h[:-1])
+    xr, sr = O.cg(A, b, tol=1e-13, restart_iterations=100000,
+                  max_iterations=5000, left=left)
+    assert sr.converged
+    assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+    _, s64 = O.cg(A, b, tol=1e-11, restart_iterations=100000,
+                  max_iterations=5000, left=left)
+    assert s.inner_iterations <= 1.15 * s64.computed_iterations + 5, (
+        s.inner_iterations, s64.computed_iterations)
+
+
+def test_cg_reliable_budget_and_exact_start(O, spk):
+    A = O.poisson_3d(12, 11, 10)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    b = O.uniform_pm1(6, A.nrows)
+    s, x = _relcg(spk, Ad, b, spk.jacobi(Ad), tol=1e-12, delta=1e-9,
+                  max_inner=7)
+    assert not s.converged and s.inner_iterations == 7
+    # the partial correction is kept: the residual went down
+    assert O.true_relres(A, b, x) < 1.0
+    xt = O.uniform_pm1(3, A.nrows)
+    bt = O.spmv(A, xt)
+    xd = spk.DeviceVector.from_numpy(xt)
+    cfg = spk.IrConfig(tolerance=1e-10, inner_method=spk.IR_CG_RELIABLE)
+    s = spk.ir_solve(spk.DeviceVector.from_numpy(bt), xd, cfg, Ad, spk.jacobi(Ad))
+    assert s.converged and s.outer_iterations == 0 and s.inner_iterations == 0
+    with pytest.raises(spk.InvalidArgument):
+        spk.ir_solve(spk.DeviceVector.from_numpy(bt), xd,
+                     spk.IrConfig(inner_method=7), Ad)

@@ -208,3 +208,18 @@ def test_tune_layout(spk):
     assert tried[0][0] == 1 and chosen in [p for p, _ in tried]
     assert Ad.panels == chosen
     assert dict(tried)[chosen] == min(us for _, us in tried)
+
+
+def test_symmetric_part_bit_exact(O, spk):
+    """(M + M^T)/2 on the device: the oracle's values bit for bit; a
+    structurally nonsymmetric pattern is rejected."""
+    for A in (O.convdiff_centered_2d(17, 13, 1.0, 0.3, 0.2, 1.0),
+              O.aniso_q1_2d(24, 5, 1.5)):
+        S = spk.CsrMatrix.from_oracle(A).symmetric_part()
+        rp, ci, v = S.download()
+        R = O.symmetric_part(A)
+        assert np.array_equal(rp, R.row_ptr) and np.array_equal(ci, R.col_ind)
+        assert np.array_equal(v, R.vals)
+    M = O.make_matrix(3, 3, [0, 0, 1, 2], [0, 1, 1, 2], [1., 2., 3., 4.])
+    with pytest.raises(spk.InvalidArgument):
+        spk.CsrMatrix.from_oracle(M).symmetric_part()
