@@ -302,6 +302,11 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                         inner_method=spk.IR_CG_RELIABLE)
     run("jacobi_relcg_fp32", lambda x: spk.ir_solve(b, x, rcfg, A, J))
     run("learned_sym_relcg_fp32", lambda x: spk.ir_solve(b, x, rcfg, A, Ms))
+    rcfg1 = spk.IrConfig(tolerance=1e-8, max_outer=200, inner_tolerance=1e-1,
+                         restart=0, max_inner=50000,
+                         inner_method=spk.IR_CG_RELIABLE)
+    run("learned_sym_relcg_fp32_update_every_10x",
+        lambda x: spk.ir_solve(b, x, rcfg1, A, Ms))
     F, t_ilu = _timed(lambda: spk.ilu0(A))
     res["ilu0_setup_ms"] = round(t_ilu, 2)
     res["ilu0_apply_ms"] = ilu_schedules(spk, F, n)
@@ -467,7 +472,7 @@ def main():
                          "spai (learned SPAI), "
                          "batched (batched PCG); "
                          "'none' for neither")
-    ap.add_argument("--spai-epochs", type=int, default=8)
+    ap.add_argument("--spai-epochs", type=int, default=24)
     ap.add_argument("--spai-train", type=int, default=512)
     ap.add_argument("--spai-side", type=int, default=1024)
     ap.add_argument("--mp-grid", type=int, default=128)

@@ -21,6 +21,12 @@
 // the flag.
 // HBM traffic per iteration: B_spmv + 10 s n bytes (K2 reads r,q,D^-1 and
 // writes r; K3 reads x,r,D^-1,p and writes x,p).
+// With an explicit approximate inverse M as left preconditioner the same
+// shape holds with one more SpMV: K2 updates r and |r|; z = M r runs as an
+// SpMV whose epilogue stores z and sums <r,z> (finisher: beta, rz); the
+// last pass does x += alpha p; p = z + beta p.  (Forming r - alpha q inside
+// that SpMV's gather instead of K2 was measured slower: two loads per
+// gathered column cost more than the pass and launch they save.)
 #include "spk_solver_common.hpp"
 
 namespace spk {
@@ -64,6 +70,23 @@ struct EpiCgRestartJacobi {  // r = b - A x; z = D^-1 r; p = z; <r, z>
     return (double)ri * (double)zi;
   }
   __device__ void finalize(double rz) const { st->rz = rz; }
+};
+
+template <typename T>
+struct EpiCgPrecDot {  // z = M r (explicit approximate inverse), <r, z>
+  static constexpr bool kReduce = true;
+  const T* __restrict__ r;
+  T* __restrict__ z;
+  CgState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    z[i] = s;
+    return (double)r[i] * (double)s;
+  }
+  __device__ void finalize(double rz) const {
+    st->beta = rz / st->rz;
+    st->rz = rz;
+  }
 };
 
 template <typename T>
@@ -137,6 +160,7 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
 
   const bool jac = is_jacobi(ML);
   const T* invd = jac ? ML->inv_diag<T>() : nullptr;
+  const bool spmv_left = ML && ML->kind == spk_precond_s::kSpmv;
   const int* done = &S->done;
 
   auto issue = [&](int64_t k) {
@@ -180,6 +204,25 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         const T pi = p[i];
         x[i] = (T)S->alpha * pi + x[i];
         p[i] = invd[i] * r[i] + (T)S->beta * pi;
+      });
+    } else if (spmv_left && right_id) {
+      // compute_new_residual + is_converged (:248-276)
+      launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
+        const T a = (T)S->alpha;
+        T ri = r[i] - a * q[i];
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+      }, [=] __device__(const double* t) { cg_check(S, t[0]); }, done);
+      // z = M r with rz_new = <r, z> in the SpMV epilogue (:278-292)
+      launch_spmv(*ML->M, (const T*)r, EpiCgPrecDot<T>{r, z, S, done}, s);
+      // compute_new_solution (:241-246) + the direction update, gated on
+      // the step count like the Jacobi path's
+      const int64_t step = k + 1;
+      launch_map(n, [=] __device__(int64_t i) {
+        if (S->iters != step) return;
+        const T pi = p[i];
+        x[i] = (T)S->alpha * pi + x[i];
+        p[i] = z[i] + (T)S->beta * pi;
       });
     } else if (jac) {
       // compute_new_solution/residual + is_converged + the z, rz_new of

@@ -321,3 +321,47 @@ def test_gmres_with_approximate_ilu(O, spk, k):
     icfg = spk.IrConfig(1e-10, 20, 1e-4, 50, 500, spk.IR_GMRES, spk.ORTHO_CGS2)
     s2 = spk.ir_solve(spk.DeviceVector.from_numpy(b), x2, icfg, Ad, F)
     assert s2.converged and O.true_relres(A, b, x2.numpy()) < 1e-10
+
+
+def _neumann_inverse(O, A):
+    """An SPD approximate inverse on A's pattern: D^-1 - (1/2) D^-1 (A - D)
+    D^-1 (first Neumann term, halved), for a symmetric diagonally dominant
+    A."""
+    n = A.nrows
+    d = np.array([A.at(i, i) for i in range(n)])
+    v = np.empty(A.nnz)
+    for i in range(n):
+        for k in range(A.row_ptr[i], A.row_ptr[i + 1]):
+            j = A.col_ind[k]
+            v[k] = 1.0 / d[i] if i == j else -0.5 * A.vals[k] / (d[i] * d[j])
+    return O.Csr(n, n, A.row_ptr.copy(), A.col_ind.copy(), v)
+
+
+@pytest.mark.parametrize("max_it", [1, 5, 6, 11, 400])
+def test_cg_with_explicit_approximate_inverse(O, spk, max_it):
+    """CG with an explicit approximate inverse in the left slot (z = M r as
+    an SpMV whose epilogue sums <r,z>; the x update rides in the direction
+    pass) follows the oracle's conjugate_gradient with the same operator,
+    also when stopped mid-way (restarts every 5 steps)."""
+    A = O.aniso_q1_2d(20, 3, 1.5)
+    M = _neumann_inverse(O, A)
+    b = O.uniform_pm1(13, A.nrows)
+    x0 = O.uniform_pm1(14, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    P = spk.spai(spk.CsrMatrix.from_oracle(M))
+    xd = dv(spk, x0)
+    restart = 5 if max_it < 400 else 1000
+    cfg = spk.CGConfig(1e-10, restart, max_it, True, spk.F64)
+    s = spk.conjugate_gradient(dv(spk, b), xd, cfg, Ad, P, spk.identity)
+    xr, sr = O.cg(A, b, x0, tol=1e-10, restart_iterations=restart,
+                  max_iterations=max_it, left=("csr", M), right=None)
+    x = xd.numpy()
+    if max_it < 400:
+        assert not s.converged and s.computed_iterations == max_it
+        assert np.allclose(x, xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+        assert np.allclose(s.iteration_residuals, sr.iteration_residuals,
+                           rtol=1e-9)
+    else:
+        assert s.converged and sr.converged
+        assert abs(s.computed_iterations - sr.computed_iterations) <= 1
+        assert O.true_relres(A, b, x) < 1e-9
