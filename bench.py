@@ -235,7 +235,7 @@ def mixed_precision_section(spk, grid=128, seed=42):
 
 
 def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
-                         epochs=8, lr=2e-3, seed=4242):
+                         epochs=24, lr=2e-3, seed=4242, pcg_only=False):
     """Learned SPAI in FGMRES(30) on 2D anisotropic variable-coefficient
     diffusion (Q1, n_side^2 interior nodes, conductivity lognormal(0,1.5),
     anisotropy log-uniform [1,1e3], random rotation), trained on the device
@@ -307,6 +307,12 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                          inner_method=spk.IR_CG_RELIABLE)
     run("learned_sym_relcg_fp32_update_every_10x",
         lambda x: spk.ir_solve(b, x, rcfg1, A, Ms))
+    jt = res["jacobi_pcg_fp64"]["ms"]
+    for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32",
+              "learned_sym_relcg_fp32_update_every_10x"):
+        res[f"speedup_{k}_vs_jacobi_pcg"] = round(jt / res[k]["ms"], 3)
+    if pcg_only:  # the PCG comparisons only (no ILU(0)-PCG, no FGMRES)
+        return res
     F, t_ilu = _timed(lambda: spk.ilu0(A))
     res["ilu0_setup_ms"] = round(t_ilu, 2)
     res["ilu0_apply_ms"] = ilu_schedules(spk, F, n)
@@ -323,9 +329,6 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                         inner_method=spk.IR_FGMRES,
                         orthogonalization=spk.ORTHO_CGS2)
     run("ir_fp32_fgmres30_learned", lambda x: spk.ir_solve(b, x, icfg, A, Ml))
-    jt = res["jacobi_pcg_fp64"]["ms"]
-    for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32"):
-        res[f"speedup_{k}_vs_jacobi_pcg"] = round(jt / res[k]["ms"], 3)
     res["speedup_learned_ir_vs_jacobi_pcg"] = round(
         jt / res["ir_fp32_fgmres30_learned"]["ms"], 3)
     res["speedup_learned_vs_untrained_fgmres"] = round(
@@ -469,7 +472,8 @@ def main():
     ap.add_argument("--sections", default="p2d,mp,spai,batched",
                     help="extra measurements after the timed steps: "
                          "p2d (poisson_2d(256) PCG), mp (mixed precision), "
-                         "spai (learned SPAI), "
+                         "spai (learned SPAI), spai-pcg (its PCG "
+                         "comparisons only), "
                          "batched (batched PCG); "
                          "'none' for neither")
     ap.add_argument("--spai-epochs", type=int, default=24)
@@ -607,10 +611,10 @@ def main():
     if "mp" in secs:
         section("mixed_precision",
                 lambda: mixed_precision_section(spk, args.mp_grid))
-    if "spai" in secs:
+    if "spai" in secs or "spai-pcg" in secs:
         section("learned_spai", lambda: learned_spai_section(
             spk, n_side=args.spai_side, train_count=args.spai_train,
-            epochs=args.spai_epochs))
+            epochs=args.spai_epochs, pcg_only="spai" not in secs))
     if "batched" in secs:
         section("batched_pcg", lambda: batched_section(spk))
     if not args.no_cpu and "a" in args.inputs:
