@@ -551,6 +551,22 @@ def main():
              "spmv_us": round(us, 2), "spmv_GBs": round(gbs, 1),
              "spmv_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
              "spmv_bytes": bs, "spmv_layout": inp.layout}
+        panels = inp.A.panels
+        if panels > 1:
+            # rows scatter over all columns: every product is a random
+            # gather, bounded by the gather rate at the layout's x window
+            w = -(-inp.A.ncols // panels)
+            gps, _ = spk.gather_rate(w)
+            gps_full, _ = spk.gather_rate(inp.A.ncols)
+            ach = inp.nnz / (us * 1e-6)
+            r["gather_ceiling"] = {
+                "window_doubles": w, "ceiling_Ggathers_s": round(gps / 1e9, 2),
+                "ceiling_full_x_Ggathers_s": round(gps_full / 1e9, 2),
+                "spmv_Ggathers_s": round(ach / 1e9, 2),
+                "frac": round(ach / gps, 4)}
+            log(f"{inp.name}: {ach / 1e9:.1f} G gathers/s against a "
+                f"{gps / 1e9:.1f} G/s ceiling at a {w}-entry window "
+                f"({gps_full / 1e9:.1f} G/s over all of x)")
         results.append(r)
         log(f"{inp.name}: spmv {us:.1f} us {gbs:.0f} GB/s "
             f"({gbs / HBM_PEAK_GBS:.1%}), relres {relres:.2e}")

@@ -71,6 +71,17 @@ spk_status spk_memset(void* dst, int value, size_t bytes);
 spk_status spk_timer_start(void);
 spk_status spk_timer_stop(float* ms);
 
+/* Gather-throughput ceiling (no reference counterpart; a measurement for
+ * the roofline of gather-bound SpMVs): streams `count` (int32 index, FP64
+ * value) pairs with nontemporal loads and multiplies each value by
+ * x[index], indices uniform (splitmix64) over a window of `window` FP64
+ * entries, 2048 workgroups of 512 threads; returns the best of `reps`
+ * timed launches as gathers per second and streamed bytes (12 per pair)
+ * per second.  Allocates and frees its buffers (count * 12 + window * 8
+ * bytes); count a multiple of 4096, window in [1, 2^31). */
+spk_status spk_gather_rate(int64_t window, int64_t count, int reps,
+                           double* gathers_per_s, double* stream_bytes_per_s);
+
 /* --------------------------------------------------------- CSR store */
 
 /* Replaces Compressed_row_matrix<T>(Compressed_row_sparsity, std::vector<T>)

@@ -38,7 +38,7 @@ __all__ = [
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig",
     "true_relres",
-    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer",
+    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB", "IR_CG_RELIABLE",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
@@ -79,6 +79,15 @@ class Timer:
         ms = C.c_float(0)
         call("spk_timer_stop", C.byref(ms))
         return ms.value
+
+
+def gather_rate(window: int, count: int = 64 << 20, reps: int = 3):
+    """Gather-throughput ceiling: (gathers/s, streamed bytes/s) of a stream
+    of (int32 index, FP64 value) pairs gathering from a window of x."""
+    g, b = C.c_double(), C.c_double()
+    call("spk_gather_rate", int(window), int(count), int(reps), C.byref(g),
+         C.byref(b))
+    return g.value, b.value
 
 
 class DeviceVector:

@@ -114,6 +114,8 @@ _SIGS = {
     "spk_memset": ([vp, i32, C.c_size_t], None),
     "spk_timer_start": ([], None),
     "spk_timer_stop": ([C.POINTER(C.c_float)], None),
+    "spk_gather_rate": ([C.c_int64, C.c_int64, C.c_int,
+                         C.POINTER(C.c_double), C.POINTER(C.c_double)], None),
     "spk_csr_create": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)], None),
     "spk_csr_create_device": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)],
                               None),

@@ -223,3 +223,16 @@ def test_symmetric_part_bit_exact(O, spk):
     M = O.make_matrix(3, 3, [0, 0, 1, 2], [0, 1, 1, 2], [1., 2., 3., 4.])
     with pytest.raises(spk.InvalidArgument):
         spk.CsrMatrix.from_oracle(M).symmetric_part()
+
+
+def test_gather_rate_ceiling(spk):
+    """The gather-throughput measurement: a window that fits the L1 gathers
+    faster than one spread over 32 MB; arguments are checked."""
+    small, sb = spk.gather_rate(2048, 8 << 20, 2)
+    large, lb = spk.gather_rate(4 << 20, 8 << 20, 2)
+    assert small > large > 1e9
+    assert abs(sb / small - 12.0) < 1e-9
+    with pytest.raises(spk.InvalidArgument):
+        spk.gather_rate(0, 4096, 1)
+    with pytest.raises(spk.InvalidArgument):
+        spk.gather_rate(1024, 4095, 1)
