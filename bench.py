@@ -337,6 +337,51 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     return res
 
 
+def krylov_section(spk, grid=128, seed=7):
+    """The other Krylov solvers on the device, FP64 + Jacobi to a 1e-8
+    relative residual: nonsymmetric 3D upwind convection-diffusion grid^3
+    (Peclet as the mixed-precision section) with BiCGSTAB, CGS, BiCG and
+    QMR; the SPD 3D Poisson grid^3 with CG and MINRES.  Times are whole
+    solves (warm), counts are the solvers' own (matvecs for the two-sided
+    methods)."""
+    pe = 10.0 + 90.0 * ((_mix(42 ^ 0x5045, 0) >> 11) / 2.0 ** 53)
+    h = 1.0 / (grid + 1)
+    v = np.array([1.0, 0.5, 0.25])
+    c = pe * h * v / np.linalg.norm(v)
+    res = {}
+    for name, A in ((f"3D upwind convection-diffusion {grid}^3",
+                     spk.convdiff_3d(grid, grid, grid, 1.0, *c)),
+                    (f"3D 7-point Poisson {grid}^3", spk.poisson_3d(grid, grid, grid))):
+        n = A.nrows
+        b = spk.uniform_pm1(seed, n)
+        J = spk.jacobi(A)
+        At = A.transpose()
+        cfg = spk.BicgstabConfig(1e-8, 20000)
+        if name.startswith("3D upwind"):
+            runs = (("bicgstab", lambda x: spk.bicgstab(b, x, cfg, A, J)),
+                    ("cgs", lambda x: spk.cgs(b, x, cfg, A, J)),
+                    ("bicg", lambda x: spk.bicg(b, x, cfg, A, At, J)),
+                    ("qmr", lambda x: spk.qmr(b, x, cfg, A, At, J)))
+        else:
+            cgc = spk.CGConfig(1e-8, 20000, 20000)
+            runs = (("cg", lambda x: spk.conjugate_gradient(b, x, cgc, A, J)),
+                    ("minres", lambda x: spk.minres(b, x, cfg, A, J)))
+        out = {"rows": n, "nnz": A.nnz}
+        for sname, fn in runs:
+            x = spk.DeviceVector.zeros(n)
+            fn(x)
+            spk._lib.call("spk_memset", x.ptr, 0, x.nbytes)
+            s, ms = _timed(lambda: fn(x))
+            rel = spk.true_relres(A, b, x)
+            out[sname] = {"ms": round(ms, 2), "iterations": s.computed_iterations,
+                          "converged": s.converged, "true_relres": rel,
+                          "breakdown": s.breakdown}
+            log(f"krylov: {name}: {sname} {s.computed_iterations} it "
+                f"{ms:.1f} ms relres {rel:.2e}")
+        res[name] = out
+    return res
+
+
 def poisson2d_rhs(spk, n, seed=42):
     """splitmix64 uniform[-1,1] minus its mean: orthogonal to the constant
     null vector of the Neumann poisson_2d (matgen.hpp:211)."""
@@ -469,12 +514,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plain CSR layout (no column panels)")
-    ap.add_argument("--sections", default="p2d,mp,spai,batched",
+    ap.add_argument("--sections", default="p2d,mp,spai,batched,krylov",
                     help="extra measurements after the timed steps: "
                          "p2d (poisson_2d(256) PCG), mp (mixed precision), "
                          "spai (learned SPAI), spai-pcg (its PCG "
                          "comparisons only), "
-                         "batched (batched PCG); "
+                         "batched (batched PCG), krylov (BiCGSTAB, CGS, "
+                         "BiCG, QMR, CG, MINRES); "
                          "'none' for neither")
     ap.add_argument("--spai-epochs", type=int, default=24)
     ap.add_argument("--spai-train", type=int, default=512)
@@ -633,6 +679,8 @@ def main():
             epochs=args.spai_epochs, pcg_only="spai" not in secs))
     if "batched" in secs:
         section("batched_pcg", lambda: batched_section(spk))
+    if "krylov" in secs:
+        section("krylov_solvers", lambda: krylov_section(spk))
     if not args.no_cpu and "a" in args.inputs:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)

@@ -659,4 +659,60 @@ Summary<T> bicg(BIter bfirst, BIter blast, XIter xfirst, XIter xlast, Config cfg
                              });
 }
 
+// minres (minres.hpp:364-395); Minres_config has Bicgstab_config's fields
+template <typename T>
+using Minres_config = Bicgstab_config<T>;
+template <typename Config, typename Left, typename Right,
+          typename T = detail::value_of<Config>>
+Summary<T> minres(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
+                  const Device_csr& A, const Left& M_left, const Right& M_right) {
+  auto c = detail::make_config<T>(cfg.tolerance, 0, cfg.max_iterations,
+                                  cfg.collect_residuals);
+  return detail::run_device<T>(c, [&](const spk_solver_config* pc,
+                                      spk_summary* s, double* h) {
+    return spk_minres_solve(A.handle(), detail::slot(M_left),
+                            detail::slot(M_right), pc, b.data(), x.data(), s, h);
+  });
+}
+
+template <typename BIter, typename XIter, typename Config, typename Left,
+          typename Right, typename T = detail::value_of<Config>>
+Summary<T> minres(BIter bfirst, BIter blast, XIter xfirst, XIter xlast,
+                  Config cfg, const Device_csr& A, const Left& M_left,
+                  const Right& M_right) {
+  return detail::run_host<T>(bfirst, blast, xfirst, xlast,
+                             [&](Device_vector<T>& b, Device_vector<T>& x) {
+                               return minres(b, x, cfg, A, M_left, M_right);
+                             });
+}
+
+// qmr (qmr.hpp:455-489): the transpose operator slot takes the store of
+// A^T; Qmr_config has Bicgstab_config's fields
+template <typename T>
+using Qmr_config = Bicgstab_config<T>;
+template <typename Config, typename Left, typename Right,
+          typename T = detail::value_of<Config>>
+Summary<T> qmr(const Device_vector<T>& b, Device_vector<T>& x, Config cfg,
+               const Device_csr& A, const Device_csr& At, const Left& M_left,
+               const Right& M_right) {
+  auto c = detail::make_config<T>(cfg.tolerance, 0, cfg.max_iterations,
+                                  cfg.collect_residuals);
+  return detail::run_device<T>(c, [&](const spk_solver_config* pc,
+                                      spk_summary* s, double* h) {
+    return spk_qmr_solve(A.handle(), At.handle(), detail::slot(M_left),
+                         detail::slot(M_right), pc, b.data(), x.data(), s, h);
+  });
+}
+
+template <typename BIter, typename XIter, typename Config, typename Left,
+          typename Right, typename T = detail::value_of<Config>>
+Summary<T> qmr(BIter bfirst, BIter blast, XIter xfirst, XIter xlast, Config cfg,
+               const Device_csr& A, const Device_csr& At, const Left& M_left,
+               const Right& M_right) {
+  return detail::run_host<T>(bfirst, blast, xfirst, xlast,
+                             [&](Device_vector<T>& b, Device_vector<T>& x) {
+                               return qmr(b, x, cfg, A, At, M_left, M_right);
+                             });
+}
+
 }  // namespace sparkit::hip

@@ -317,6 +317,25 @@ spk_status spk_bicg_solve(spk_csr A, spk_csr At, spk_precond M_left,
                           const void* d_b, void* d_x, spk_summary* summary,
                           double* history);
 
+/* minres(b, x, cfg, A, M_L, M_R) (minres.hpp:80-395): Lanczos with the
+ * combined preconditioner M_R(M_L(.)), Givens QR, d-vector updates; the
+ * residual estimate |phi_bar| is tested against ||b|| (:313-320); one
+ * iteration per matvec.  rho == 0 in the QR update stops with
+ * SPK_BREAKDOWN.  cfg->restart is ignored. */
+spk_status spk_minres_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
+                            const spk_solver_config* cfg, const void* d_b,
+                            void* d_x, spk_summary* summary, double* history);
+
+/* qmr(b, x, cfg, A, A^T, M_L, M_R) (qmr.hpp:86-491): QMR without
+ * look-ahead, the transpose operator as the store At; counts matvecs (two
+ * per step, :251); the residual estimate is |tau|.  delta == 0 (the
+ * reference repeats the step forever) and a zero Lanczos norm without
+ * convergence stop with SPK_BREAKDOWN.  cfg->restart is ignored. */
+spk_status spk_qmr_solve(spk_csr A, spk_csr At, spk_precond M_left,
+                         spk_precond M_right, const spk_solver_config* cfg,
+                         const void* d_b, void* d_x, spk_summary* summary,
+                         double* history);
+
 /* Mixed-precision iterative refinement (no reference counterpart; the
  * closest structure is the outer correction loop of richardson.hpp:112):
  *   r = b - A x in FP64; stop when ||r|| / ||b|| < tolerance (FP64 true

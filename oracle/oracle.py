@@ -116,7 +116,7 @@ def _declare(L):
     L.orc_splitmix64_at.argtypes = [C.c_uint64, C.c_uint64]
     L.orc_splitmix64_at.restype = C.c_uint64
     L.orc_uniform_pm1.argtypes = [C.c_uint64, C.c_int64, _f64p]
-    for nm in ("orc_cg", "orc_gmres", "orc_bicgstab", "orc_cgs"):
+    for nm in ("orc_cg", "orc_gmres", "orc_bicgstab", "orc_cgs", "orc_minres"):
         for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
                                    ("f", _f32p, ConfigF, SummaryF)):
             getattr(L, f"{nm}_{sfx}").argtypes = [
@@ -166,9 +166,10 @@ def _declare(L):
                                        _f64p, _f64p, _f64p]
     for sfx, fp, cfg, summ in (("d", _f64p, ConfigD, SummaryD),
                                ("f", _f32p, ConfigF, SummaryF)):
-        getattr(L, f"orc_bicg_{sfx}").argtypes = [
-            C.c_int64, fp, fp, C.POINTER(cfg)] + [C.c_void_p] * 8 + [
-            C.POINTER(summ), C.c_void_p]
+        for nm in ("orc_bicg", "orc_qmr"):
+            getattr(L, f"{nm}_{sfx}").argtypes = [
+                C.c_int64, fp, fp, C.POINTER(cfg)] + [C.c_void_p] * 8 + [
+                C.POINTER(summ), C.c_void_p]
     L.orc_is_valid_permutation.argtypes = [C.c_int64, _i64p]
     L.orc_csr_permute.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p,
                                   C.c_void_p, _i64p, C.c_int, _i64p, _i64p,
@@ -462,10 +463,10 @@ def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
     hist = np.zeros(max(max_iterations, 1), dtype)
     a_op = _Op(("csr", A), A, dtype)
     ml = _Op(left, A, dtype)
-    if name == "bicg":
+    if name in ("bicg", "qmr"):
         at = _Op(("csrT", A), A, dtype)
         mr = _Op(right, A, dtype)
-        getattr(L, f"orc_bicg_{sfx}")(n, b, x, C.byref(cfg), a_op.fn, a_op.ctx,
+        getattr(L, f"orc_{name}_{sfx}")(n, b, x, C.byref(cfg), a_op.fn, a_op.ctx,
                                       at.fn, at.ctx, ml.fn, ml.ctx, mr.fn,
                                       mr.ctx, C.byref(summ), _ptr(hist))
     elif name == "fgmres":
@@ -508,6 +509,19 @@ def bicg(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
          dtype=np.float64):
     """bicg(): bicg.hpp:86-397, transpose operator multiply_transpose."""
     return _solve("bicg", A, b, x0, tol, 0, max_iterations, left, right, dtype)
+
+
+def minres(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
+           dtype=np.float64):
+    """minres(): minres.hpp:80-395."""
+    return _solve("minres", A, b, x0, tol, 0, max_iterations, left, right,
+                  dtype)
+
+
+def qmr(A, b, x0=None, *, tol, max_iterations, left=None, right=None,
+        dtype=np.float64):
+    """qmr(): qmr.hpp:86-491, transpose operator multiply_transpose."""
+    return _solve("qmr", A, b, x0, tol, 0, max_iterations, left, right, dtype)
 
 
 def spmv_transpose(A, x):

@@ -289,6 +289,26 @@ void orc_cgs_f(int64_t n, const float* b, float* x, const orc_config_f* cfg,
                orc_op_f MR, const void* mrctx, orc_summary_f* out,
                float* history);
 
+/* MinresSolver: minres.hpp:80-395 (M = M_L M_R, one matvec per step) */
+void orc_minres_d(int64_t n, const double* b, double* x,
+                  const orc_config_d* cfg, orc_op_d A, const void* actx,
+                  orc_op_d ML, const void* mlctx, orc_op_d MR,
+                  const void* mrctx, orc_summary_d* out, double* history);
+void orc_minres_f(int64_t n, const float* b, float* x, const orc_config_f* cfg,
+                  orc_op_f A, const void* actx, orc_op_f ML, const void* mlctx,
+                  orc_op_f MR, const void* mrctx, orc_summary_f* out,
+                  float* history);
+
+/* QmrSolver: qmr.hpp:86-491; At is the transpose operator */
+void orc_qmr_d(int64_t n, const double* b, double* x, const orc_config_d* cfg,
+               orc_op_d A, const void* actx, orc_op_d At, const void* atctx,
+               orc_op_d ML, const void* mlctx, orc_op_d MR, const void* mrctx,
+               orc_summary_d* out, double* history);
+void orc_qmr_f(int64_t n, const float* b, float* x, const orc_config_f* cfg,
+               orc_op_f A, const void* actx, orc_op_f At, const void* atctx,
+               orc_op_f ML, const void* mlctx, orc_op_f MR, const void* mrctx,
+               orc_summary_f* out, float* history);
+
 /* BicgSolver: bicg.hpp:86-397; At is the transpose operator */
 void orc_bicg_d(int64_t n, const double* b, double* x, const orc_config_d* cfg,
                 orc_op_d A, const void* actx, orc_op_d At, const void* atctx,

@@ -36,7 +36,7 @@ __all__ = [
     "extract_diagonal", "jacobi_apply", "ilu_apply", "CGConfig",
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
-    "bicg", "BicgConfig",
+    "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
     "true_relres",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
@@ -693,6 +693,29 @@ def cgs(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
     c = _cfg(cfg, 0)
     return _run("spk_cgs_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
                                   b.ptr, x.ptr), cfg)
+
+
+MinresConfig = BicgstabConfig  # Minres_config (minres.hpp:24-45)
+QmrConfig = BicgstabConfig  # Qmr_config (qmr.hpp:24-42)
+
+
+def minres(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,
+           M_right=identity) -> Summary:
+    """minres() (minres.hpp:364-395) for symmetric A and M = M_L M_R."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, 0)
+    return _run("spk_minres_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
+                                     b.ptr, x.ptr), cfg)
+
+
+def qmr(b, x, cfg: BicgstabConfig, A: CsrMatrix, At: CsrMatrix,
+        M_left=identity, M_right=identity) -> Summary:
+    """qmr() (qmr.hpp:455-489) with the transpose operator given as the
+    store At = A.transpose(); counts matrix-vector products."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, 0)
+    return _run("spk_qmr_solve", (A.h, At.h, _h(M_left), _h(M_right),
+                                  C.byref(c), b.ptr, x.ptr), cfg)
 
 
 def bicgstab(b, x, cfg: BicgstabConfig, A: CsrMatrix, M_left=identity,

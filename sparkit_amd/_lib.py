@@ -162,6 +162,10 @@ _SIGS = {
                        C.POINTER(Summary), vp], None),
     "spk_bicg_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                         C.POINTER(Summary), vp], None),
+    "spk_minres_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                          C.POINTER(Summary), vp], None),
+    "spk_qmr_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                       C.POINTER(Summary), vp], None),
     "spk_csr_transpose": ([vp, C.POINTER(vp)], None),
     "spk_csr_symmetric_part": ([vp, C.POINTER(vp)], None),
     "spk_true_relres": ([vp, vp, vp, P_dbl], None),

@@ -17,13 +17,24 @@ enum : int {
   kWsMisc = 60,
   kWsCgs = 64,       // 64..74
   kWsBicg = 75,      // 75..84
+  kWsMinres = 85,    // 85..94
+  kWsQmr = 95,       // 95..109
 };
+
 
 inline bool is_identity(const spk_precond_s* M) {
   return M == nullptr || M->kind == spk_precond_s::kIdentity;
 }
 inline bool is_jacobi(const spk_precond_s* M) {
   return M != nullptr && M->kind == spk_precond_s::kJacobi;
+}
+// The diagonal of a Jacobi preconditioner, nullptr for the identity.
+template <typename T>
+inline const T* diag_of(spk_precond_s* M) {
+  return is_jacobi(M) ? M->inv_diag<T>() : nullptr;
+}
+inline bool is_diagonal(const spk_precond_s* M) {
+  return is_identity(M) || is_jacobi(M);
 }
 
 template <typename T>

@@ -293,6 +293,28 @@ void test_cgs_bicg() {
   for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
 }
 
+void test_minres_qmr() {
+  // minres_test.cpp:124-144 / qmr_test.cpp:139-173
+  auto Ah = tridiagonal(4, -1.0, 4.0, -1.0);
+  auto A = sh::Device_csr::from(Ah);
+  std::vector<double> xt{1, 2, 3, 4};
+  auto b = host_multiply(Ah, xt);
+  std::vector<double> x(4, 0.0);
+  sh::Minres_config<double> cfg{1e-12, 100, false};
+  auto s = sh::minres(b.begin(), b.end(), x.begin(), x.end(), cfg, A,
+                      sh::Device_identity{}, sh::Device_identity{});
+  CHECK(s.converged);
+  for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
+  std::fill(x.begin(), x.end(), 0.0);
+  auto At = sh::transpose(A);
+  sh::Qmr_config<double> qcfg{1e-12, 100, false};
+  auto s2 = sh::qmr(b.begin(), b.end(), x.begin(), x.end(), qcfg, A, At,
+                    sh::jacobi(A), sh::Device_identity{});
+  CHECK(s2.converged);
+  CHECK(s2.computed_iterations % 2 == 0);
+  for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
+}
+
 void test_learned_spai_and_batched() {
   // a 9-point anisotropic diffusion matrix from the device generator
   spk_csr g = nullptr;
@@ -341,6 +363,7 @@ int main() {
                {"device_vectors_and_fp32", test_device_vectors_and_fp32},
                {"errors", test_errors},
                {"cgs_bicg", test_cgs_bicg},
+               {"minres_qmr", test_minres_qmr},
                {"learned_spai_and_batched", test_learned_spai_and_batched}};
   for (const auto& c : cases) {
     int before = g_failures;
