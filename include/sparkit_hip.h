@@ -38,7 +38,8 @@ enum {
   SPK_OUT_OF_MEMORY = 3,
   SPK_HIP_ERROR = 4,
   SPK_BREAKDOWN = 5, /* numerical breakdown the reference leaves undetected */
-  SPK_NO_DEVICE = 6
+  SPK_NO_DEVICE = 6,
+  SPK_RUNTIME_ERROR = 7 /* std::runtime_error (matrix_market.hpp) */
 };
 
 /* Arithmetic type of a store, vector or solve. */
@@ -134,6 +135,26 @@ spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
  * multiply_transpose(A, x) (sparse_blas.hpp:36-52) with each output summed
  * in the reference's order. */
 spk_status spk_csr_transpose(spk_csr A, spk_csr* out);
+/* Matrix Market files (matrix_market.hpp:43-163, matrix_market.cpp:28-133).
+ * Reading follows read_matrix_market + to_compressed_row: banner checks
+ * (coordinate format only, no complex field: SPK_RUNTIME_ERROR), '%'
+ * comment lines, 1-based indices, pattern entries = 1, symmetric files
+ * mirror their off-diagonal entries (skew-symmetric and hermitian are read
+ * as written, as the reference does), a repeated position keeps the last
+ * entry; the store is assembled on the device (spk_csr_assemble).
+ * Writing follows write_matrix_market: a "coordinate real general" banner,
+ * "rows cols nnz", then "row col value" per entry in storage order with
+ * the value printed like std::ostream's default (%g, 6 significant
+ * digits).  The _text variants work on memory: for writing, *len receives
+ * the text's length and the text (NUL-terminated) is copied when buf has
+ * room for len + 1 bytes. */
+spk_status spk_csr_read_matrix_market(const char* path, spk_csr* out);
+spk_status spk_csr_read_matrix_market_text(const char* text, size_t len,
+                                           spk_csr* out);
+spk_status spk_csr_write_matrix_market(spk_csr A, const char* path);
+spk_status spk_csr_write_matrix_market_text(spk_csr A, char* buf, size_t cap,
+                                            size_t* len);
+
 /* (M + M^T) / 2 on M's own pattern (no reference counterpart: the SPD part
  * of a nonsymmetric approximate inverse, usable in conjugate_gradient's
  * preconditioner slot).  The pattern must be structurally symmetric

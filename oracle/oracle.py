@@ -678,6 +678,71 @@ def symmetric_part(M: Csr) -> Csr:
     return Csr(n, M.ncols, M.row_ptr.copy(), M.col_ind.copy(), v)
 
 
+def _mm_int(tok):
+    """`is >> size_type` on one token: the leading integer, else 0."""
+    import re
+    m = re.match(r"[+-]?\d+", tok)
+    return int(m.group(0)) if m else 0
+
+
+def read_matrix_market_text(text):
+    """read_matrix_market (matrix_market.hpp:43-104) + to_compressed_row:
+    (rows, cols, {(i, j): value}) with the last entry at a position kept
+    (Coordinate_matrix::insert_or_assign), as a Csr."""
+    lines = text.split("\n")
+    if text.endswith("\n"):
+        lines = lines[:-1]
+    it = iter(lines)
+    try:
+        banner = next(it)
+    except StopIteration:
+        raise OracleError(7, "matrix market: unexpected end of input")
+    tok = (banner.split() + [""] * 5)[:5]
+    if tok[0].lower() != "%%matrixmarket":
+        raise OracleError(7, "invalid banner header")
+    fmt, field, sym = tok[2].lower(), tok[3].lower(), tok[4].lower()
+    if fmt not in ("coordinate", "array") or field not in (
+            "real", "integer", "complex", "pattern") or sym not in (
+            "general", "symmetric", "skew-symmetric", "hermitian"):
+        raise OracleError(7, "unknown banner word")
+    if fmt != "coordinate" or field == "complex":
+        raise OracleError(7, "unsupported")
+    line = ""
+    for line in it:
+        if line and line[0] != "%":
+            break
+    t = line.split() + ["", "", ""]
+    rows, cols, nnz = _mm_int(t[0]), _mm_int(t[1]), _mm_int(t[2])
+    entries = {}
+    for _ in range(nnz):
+        e = (next(it, "").split() + ["", "", ""])
+        i, j = _mm_int(e[0]) - 1, _mm_int(e[1]) - 1
+        if field == "pattern":
+            v = 1.0
+        else:
+            try:
+                v = float(e[2]) if e[2][:1] in "+-.0123456789" and e[2] else 0.0
+            except ValueError:
+                v = 0.0
+        entries[(i, j)] = v
+        if sym == "symmetric" and i != j:
+            entries[(j, i)] = v
+    keys = list(entries)
+    return make_matrix(rows, cols, [k[0] for k in keys], [k[1] for k in keys],
+                       [entries[k] for k in keys])
+
+
+def write_matrix_market_text(A: Csr):
+    """write_matrix_market(os, A) (matrix_market.hpp:123-148): values as
+    std::ostream prints a double by default (%g)."""
+    out = ["%%MatrixMarket matrix coordinate real general",
+           f"{A.nrows} {A.ncols} {A.nnz}"]
+    for i in range(A.nrows):
+        for k in range(A.row_ptr[i], A.row_ptr[i + 1]):
+            out.append(f"{i + 1} {A.col_ind[k] + 1} {'%g' % A.vals[k]}")
+    return "\n".join(out) + "\n"
+
+
 def is_valid_permutation(perm):
     return bool(lib().orc_is_valid_permutation(
         len(perm), np.ascontiguousarray(perm, np.int64)))

@@ -22,13 +22,14 @@ path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _lib
 from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS2, ORTHO_MGS, InvalidArgument,
-                   LogicError, SpkError, call, load)
+                   LogicError, ReadError, SpkError, call, load)
 
 __all__ = [
     "init", "synchronize", "DeviceVector", "CsrMatrix", "Preconditioner",
@@ -38,7 +39,7 @@ __all__ = [
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
     "true_relres",
-    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "Timer", "gather_rate",
+    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB", "IR_CG_RELIABLE",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
@@ -208,6 +209,35 @@ class CsrMatrix:
         h = C.c_void_p()
         call("spk_csr_transpose", self.h, C.byref(h))
         return CsrMatrix(h)
+
+    @staticmethod
+    def from_matrix_market(path) -> "CsrMatrix":
+        """read_matrix_market(path) + to_compressed_row
+        (matrix_market.hpp:43-104), assembled on the device."""
+        h = C.c_void_p()
+        call("spk_csr_read_matrix_market", os.fsencode(path), C.byref(h))
+        return CsrMatrix(h)
+
+    @staticmethod
+    def from_matrix_market_text(text) -> "CsrMatrix":
+        """The same from a string (an std::istream in the reference)."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        call("spk_csr_read_matrix_market_text", data, len(data), C.byref(h))
+        return CsrMatrix(h)
+
+    def to_matrix_market_text(self) -> str:
+        """write_matrix_market(os, A) (matrix_market.hpp:123-148)."""
+        n = C.c_size_t()
+        call("spk_csr_write_matrix_market_text", self.h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        call("spk_csr_write_matrix_market_text", self.h, buf, n.value + 1,
+             C.byref(n))
+        return buf.raw[:n.value].decode()
+
+    def write_matrix_market(self, path):
+        """write_matrix_market(path, A) (matrix_market.hpp:150-160)."""
+        call("spk_csr_write_matrix_market", self.h, os.fsencode(path))
 
     def symmetric_part(self) -> "CsrMatrix":
         """(M + M^T) / 2 on M's (structurally symmetric) pattern."""

@@ -17,6 +17,7 @@ OUT_OF_MEMORY = 3
 HIP_ERROR = 4
 BREAKDOWN = 5
 NO_DEVICE = 6
+RUNTIME_ERROR = 7
 
 F64 = 0
 F32 = 1
@@ -37,6 +38,10 @@ class InvalidArgument(SpkError, ValueError):
 
 class LogicError(SpkError):
     """std::logic_error in the reference."""
+
+
+class ReadError(SpkError):
+    """std::runtime_error in the reference (matrix_market.hpp)."""
 
 
 class SolverConfig(C.Structure):
@@ -167,6 +172,12 @@ _SIGS = {
     "spk_qmr_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                        C.POINTER(Summary), vp], None),
     "spk_csr_transpose": ([vp, C.POINTER(vp)], None),
+    "spk_csr_read_matrix_market": ([C.c_char_p, C.POINTER(vp)], None),
+    "spk_csr_read_matrix_market_text": ([C.c_char_p, C.c_size_t, C.POINTER(vp)],
+                                        None),
+    "spk_csr_write_matrix_market": ([vp, C.c_char_p], None),
+    "spk_csr_write_matrix_market_text": ([vp, C.c_char_p, C.c_size_t,
+                                          C.POINTER(C.c_size_t)], None),
     "spk_csr_symmetric_part": ([vp, C.POINTER(vp)], None),
     "spk_true_relres": ([vp, vp, vp, P_dbl], None),
     "spk_ir_solve": ([vp, vp, C.POINTER(IrConfig), vp, vp,
@@ -233,6 +244,8 @@ def check(status: int, where: str):
         raise InvalidArgument(status, where, msg)
     if status == LOGIC_ERROR:
         raise LogicError(status, where, msg)
+    if status == RUNTIME_ERROR:
+        raise ReadError(status, where, msg)
     raise SpkError(status, where, msg)
 
 
