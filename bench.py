@@ -382,6 +382,51 @@ def krylov_section(spk, grid=128, seed=7):
     return res
 
 
+def mmio_section(spk, grid=96):
+    """Matrix Market text through a device store: write the 3D Poisson
+    grid^3 store as text, read it back (host scan + device assembly), and
+    check the pattern and values round-trip."""
+    A = spk.poisson_3d(grid, grid, grid)
+    t, t_w = _timed(lambda: A.to_matrix_market_text())
+    B, t_r = _timed(lambda: spk.CsrMatrix.from_matrix_market_text(t))
+    ra, ca, va = A.download()
+    rb, cb, vb = B.download()
+    same = bool(np.array_equal(ra, rb) and np.array_equal(ca, cb)
+                and np.array_equal(va, vb))
+    mb = len(t) / 1e6
+    log(f"matrix market: {A.nnz} entries, {mb:.0f} MB text, write "
+        f"{t_w:.0f} ms, read {t_r:.0f} ms, round trip exact: {same}")
+    return {"input": f"3D 7-point Poisson {grid}^3", "entries": A.nnz,
+            "text_MB": round(mb, 1), "write_ms": round(t_w, 1),
+            "read_ms": round(t_r, 1), "read_MB_s": round(mb / (t_r / 1e3), 1),
+            "round_trip_exact": same}
+
+
+def rcm_section(spk, grid=128, seed=5):
+    """Reverse Cuthill-McKee on the device: the 3D Poisson grid^3 under a
+    seeded random relabelling (a scrambled band), reordered by RCM and
+    permuted back with dperm; bandwidth before and after."""
+    A0 = spk.poisson_3d(grid, grid, grid)
+    n = A0.nrows
+    lab = np.random.default_rng(seed).permutation(n)
+    A = A0.permute(lab, "both")
+
+    def band(M):
+        rp, ci, _ = M.download()
+        rows = np.repeat(np.arange(M.nrows), np.diff(rp))
+        return int(np.abs(ci - rows).max())
+
+    perm, t_rcm = _timed(lambda: A.rcm())
+    B, t_perm = _timed(lambda: A.permute(perm, "both"))
+    out = {"input": f"3D 7-point Poisson {grid}^3, randomly relabelled",
+           "rows": n, "nnz": A.nnz, "rcm_ms": round(t_rcm, 1),
+           "dperm_ms": round(t_perm, 1), "bandwidth_scrambled": band(A),
+           "bandwidth_rcm": band(B), "bandwidth_natural": band(A0)}
+    log(f"rcm: {n} rows in {t_rcm:.0f} ms, bandwidth {out['bandwidth_scrambled']}"
+        f" -> {out['bandwidth_rcm']} (natural {out['bandwidth_natural']})")
+    return out
+
+
 def poisson2d_rhs(spk, n, seed=42):
     """splitmix64 uniform[-1,1] minus its mean: orthogonal to the constant
     null vector of the Neumann poisson_2d (matgen.hpp:211)."""
@@ -514,13 +559,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plain CSR layout (no column panels)")
-    ap.add_argument("--sections", default="p2d,mp,spai,batched,krylov",
+    ap.add_argument("--sections",
+                    default="p2d,mp,spai,batched,krylov,mmio,rcm",
                     help="extra measurements after the timed steps: "
                          "p2d (poisson_2d(256) PCG), mp (mixed precision), "
                          "spai (learned SPAI), spai-pcg (its PCG "
                          "comparisons only), "
                          "batched (batched PCG), krylov (BiCGSTAB, CGS, "
-                         "BiCG, QMR, CG, MINRES); "
+                         "BiCG, QMR, CG, MINRES), mmio (Matrix Market "
+                         "round trip), rcm (device reverse Cuthill-McKee); "
                          "'none' for neither")
     ap.add_argument("--spai-epochs", type=int, default=24)
     ap.add_argument("--spai-train", type=int, default=512)
@@ -681,6 +728,10 @@ def main():
         section("batched_pcg", lambda: batched_section(spk))
     if "krylov" in secs:
         section("krylov_solvers", lambda: krylov_section(spk))
+    if "mmio" in secs:
+        section("matrix_market", lambda: mmio_section(spk))
+    if "rcm" in secs:
+        section("rcm", lambda: rcm_section(spk))
     if not args.no_cpu and "a" in args.inputs:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)

@@ -135,6 +135,16 @@ spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t count,
  * multiply_transpose(A, x) (sparse_blas.hpp:36-52) with each output summed
  * in the reference's order. */
 spk_status spk_csr_transpose(spk_csr A, spk_csr* out);
+/* reverse_cuthill_mckee(A.sparsity()) (reordering.cpp:152-203) on the
+ * device: d_perm (n int64, device) receives perm[old] = new, for
+ * spk_csr_permute(A, perm, SPK_PERM_BOTH) (dperm).  Neighbours are ordered
+ * by degree with ties in column order (a stable sort, as the reference's
+ * std::sort orders rows of up to 16 neighbours).  The pseudo-peripheral
+ * start node (reordering.cpp:112-150) is exposed for tests.  Square
+ * matrices only (SPK_INVALID_ARGUMENT). */
+spk_status spk_csr_rcm(spk_csr A, int64_t* d_perm);
+spk_status spk_csr_pseudo_peripheral_node(spk_csr A, int64_t* node);
+
 /* Matrix Market files (matrix_market.hpp:43-163, matrix_market.cpp:28-133).
  * Reading follows read_matrix_market + to_compressed_row: banner checks
  * (coordinate format only, no complex field: SPK_RUNTIME_ERROR), '%'

@@ -743,6 +743,99 @@ def write_matrix_market_text(A: Csr):
     return "\n".join(out) + "\n"
 
 
+def symmetrize_pattern(A: Csr):
+    """symmetrize_pattern (reordering.cpp:19-38): the pattern of A + A^T
+    (sorted unique columns per row), returned as (row_ptr, col_ind)."""
+    n = A.nrows
+    rows = [set() for _ in range(n)]
+    for i in range(n):
+        for k in range(A.row_ptr[i], A.row_ptr[i + 1]):
+            j = int(A.col_ind[k])
+            rows[i].add(j)
+            rows[j].add(i)
+    rp = np.zeros(n + 1, np.int64)
+    ci = []
+    for i in range(n):
+        r = sorted(rows[i])
+        ci.extend(r)
+        rp[i + 1] = rp[i] + len(r)
+    return rp, np.array(ci, np.int64)
+
+
+def adjacency_degree(rp, ci):
+    """adjacency_degree (reordering.cpp:40-56): off-diagonal count per row."""
+    n = len(rp) - 1
+    return np.array([sum(1 for k in range(rp[i], rp[i + 1]) if ci[k] != i)
+                     for i in range(n)], np.int64)
+
+
+def _cm_bfs(rp, ci, deg, start, visited):
+    """One Cuthill-McKee sweep from start over unvisited nodes: each node's
+    unvisited neighbours are appended sorted by degree, ties in column order
+    (reordering.cpp:62-108, :169-191).  Returns the order and the level of
+    every appended node."""
+    order, level = [start], [0]
+    visited[start] = True
+    head = 0
+    while head < len(order):
+        u = order[head]
+        nb = [int(ci[k]) for k in range(rp[u], rp[u + 1])
+              if ci[k] != u and not visited[ci[k]]]
+        nb.sort(key=lambda v: deg[v])  # stable
+        for v in nb:
+            if not visited[v]:
+                visited[v] = True
+                order.append(v)
+                level.append(level[head] + 1)
+        head += 1
+    return order, level
+
+
+def pseudo_peripheral_node(A: Csr):
+    """pseudo_peripheral_node (reordering.cpp:112-150)."""
+    rp, ci = symmetrize_pattern(A)
+    deg = adjacency_degree(rp, ci)
+    start = int(np.argmin(deg))  # the first node of minimum degree
+
+    def levels(s):
+        order, lev = _cm_bfs(rp, ci, deg, s, np.zeros(A.nrows, bool))
+        ecc = max(lev)
+        return ecc, [v for v, l in zip(order, lev) if l == ecc]
+
+    ecc, last = levels(start)
+    while True:
+        cand = last[0]
+        for v in last[1:]:
+            if deg[v] < deg[cand]:
+                cand = v
+        new_ecc, new_last = levels(cand)
+        if new_ecc <= ecc:
+            return start
+        start, ecc, last = cand, new_ecc, new_last
+
+
+def reverse_cuthill_mckee(A: Csr):
+    """reverse_cuthill_mckee (reordering.cpp:152-203): perm[old] = new."""
+    n = A.nrows
+    rp, ci = symmetrize_pattern(A)
+    deg = adjacency_degree(rp, ci)
+    visited = np.zeros(n, bool)
+    ordering = []
+    while len(ordering) < n:
+        if not ordering:
+            start = pseudo_peripheral_node(A)
+        else:
+            cand = [i for i in range(n) if not visited[i]]
+            start = min(cand, key=lambda i: (deg[i], i))
+        order, _ = _cm_bfs(rp, ci, deg, start, visited)
+        ordering.extend(order)
+    ordering.reverse()
+    perm = np.zeros(n, np.int64)
+    for new, old in enumerate(ordering):
+        perm[old] = new
+    return perm
+
+
 def is_valid_permutation(perm):
     return bool(lib().orc_is_valid_permutation(
         len(perm), np.ascontiguousarray(perm, np.int64)))

@@ -255,6 +255,19 @@ class CsrMatrix:
         call("spk_csr_permute", self.h, p.ptr, m, C.byref(h))
         return CsrMatrix(h)
 
+    def rcm(self) -> np.ndarray:
+        """reverse_cuthill_mckee(A.sparsity()) (reordering.cpp:152-203) on
+        the device: perm[old] = new (for permute(perm, 'both'))."""
+        p = DeviceVector.zeros(self.nrows)
+        call("spk_csr_rcm", self.h, p.ptr)
+        return p.numpy().view(np.int64).copy()
+
+    def pseudo_peripheral_node(self) -> int:
+        """pseudo_peripheral_node (reordering.cpp:112-150)."""
+        v = C.c_int64()
+        call("spk_csr_pseudo_peripheral_node", self.h, C.byref(v))
+        return v.value
+
     @property
     def panels(self) -> int:
         """Number of column panels the SpMV runs over (1: plain store)."""

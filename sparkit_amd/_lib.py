@@ -172,6 +172,8 @@ _SIGS = {
     "spk_qmr_solve": ([vp, vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                        C.POINTER(Summary), vp], None),
     "spk_csr_transpose": ([vp, C.POINTER(vp)], None),
+    "spk_csr_rcm": ([vp, vp], None),
+    "spk_csr_pseudo_peripheral_node": ([vp, C.POINTER(C.c_int64)], None),
     "spk_csr_read_matrix_market": ([C.c_char_p, C.POINTER(vp)], None),
     "spk_csr_read_matrix_market_text": ([C.c_char_p, C.c_size_t, C.POINTER(vp)],
                                         None),
