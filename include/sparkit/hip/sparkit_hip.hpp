@@ -630,6 +630,28 @@ inline Device_csr transpose(const Device_csr& A) {
   return Device_csr(t);
 }
 
+// read_matrix_market + to_compressed_row / write_matrix_market
+// (matrix_market.hpp:43-160); failures throw std::runtime_error
+inline Device_csr read_matrix_market(const std::string& path) {
+  spk_csr t = nullptr;
+  check(spk_csr_read_matrix_market(path.c_str(), &t), "read_matrix_market");
+  return Device_csr(t);
+}
+inline void write_matrix_market(const std::string& path, const Device_csr& A) {
+  check(spk_csr_write_matrix_market(A.handle(), path.c_str()), "write_matrix_market");
+}
+
+// reverse_cuthill_mckee(A.sparsity()) (reordering.cpp:152-203): perm[old] = new
+inline std::vector<int64_t> reverse_cuthill_mckee(const Device_csr& A) {
+  Device_vector<double> d(A.rows());  // n 8-byte slots for the device result
+  check(spk_csr_rcm(A.handle(), reinterpret_cast<int64_t*>(d.data())),
+        "reverse_cuthill_mckee");
+  std::vector<int64_t> perm(A.rows());
+  check(spk_memcpy_dtoh(perm.data(), d.data(), perm.size() * sizeof(int64_t)),
+        "reverse_cuthill_mckee");
+  return perm;
+}
+
 // bicg (bicg.hpp:86-397): the transpose operator slot takes the store of
 // A^T; Bicg_config has Bicgstab_config's fields
 template <typename T>

@@ -315,6 +315,28 @@ void test_minres_qmr() {
   for (int i = 0; i < 4; ++i) CHECK_NEAR(x[i], xt[i], 1e-8);
 }
 
+void test_rcm_and_matrix_market() {
+  // reordering_test.cpp:144-163 (arrow matrix), matrix_market_test.cpp:186-211
+  std::vector<int64_t> rp{0, 4, 6, 8, 10}, ci{0, 1, 2, 3, 0, 1, 0, 2, 0, 3};
+  std::vector<double> v{4, 1, 1, 1, 1, 4, 1, 4, 1, 4};
+  spk_csr h = nullptr;
+  sparkit::hip::check(spk_csr_create(4, 4, 10, rp.data(), ci.data(), v.data(), &h),
+                      "create");
+  sh::Device_csr A(h);
+  auto perm = sh::reverse_cuthill_mckee(A);
+  std::vector<int> seen(4, 0);
+  for (auto p : perm) {
+    CHECK(p >= 0 && p < 4);
+    if (p >= 0 && p < 4) seen[(size_t)p] += 1;
+  }
+  for (int k : seen) CHECK(k == 1);
+  const std::string path = "/tmp/spk_cpp_api_test.mtx";
+  sh::write_matrix_market(path, A);
+  auto B = sh::read_matrix_market(path);
+  CHECK(B.rows() == 4);
+  CHECK(throws<std::runtime_error>([&] { sh::read_matrix_market("/nonexistent/x.mtx"); }));
+}
+
 void test_learned_spai_and_batched() {
   // a 9-point anisotropic diffusion matrix from the device generator
   spk_csr g = nullptr;
@@ -364,6 +386,7 @@ int main() {
                {"errors", test_errors},
                {"cgs_bicg", test_cgs_bicg},
                {"minres_qmr", test_minres_qmr},
+               {"rcm_and_matrix_market", test_rcm_and_matrix_market},
                {"learned_spai_and_batched", test_learned_spai_and_batched}};
   for (const auto& c : cases) {
     int before = g_failures;
