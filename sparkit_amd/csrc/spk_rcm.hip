@@ -177,6 +177,8 @@ struct Rcm {
   void setup(spk_csr_s& A) {
     SPK_REQUIRE(A.nrows == A.ncols, SPK_INVALID_ARGUMENT,
                 "reordering needs a square matrix");
+    SPK_REQUIRE(2 * A.nnz < INT32_MAX, SPK_INVALID_ARGUMENT,
+                "the symmetrised pattern exceeds int32 indexing");
     n = A.nrows;
     hipStream_t s = stream();
     // symmetrize_pattern (reordering.cpp:19-38) by device assembly
