@@ -37,6 +37,21 @@ struct CgState {
   int64_t iters;
   int32_t done, converged, collect, pad;
   double* history;
+  // one-kernel hand-off path: rz by step parity (a kernel reads one and
+  // writes the other), the step whose residual update ran
+  double rzb[2];
+  int64_t k2_step;
+};
+
+// launch gates of the one-kernel hand-off path
+struct GateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+struct GateStep {  // runs iff the residual update of this step ran
+  const CgState* st;
+  int64_t step;
+  __device__ bool operator()() const { return st->k2_step == step; }
 };
 
 template <typename T>
@@ -69,7 +84,11 @@ struct EpiCgRestartJacobi {  // r = b - A x; z = D^-1 r; p = z; <r, z>
     p[i] = zi;
     return (double)ri * (double)zi;
   }
-  __device__ void finalize(double rz) const { st->rz = rz; }
+  __device__ void finalize(double rz) const {
+    st->rz = rz;
+    st->rzb[par] = rz;
+  }
+  int par;  // step parity (the one-kernel hand-off path)
 };
 
 template <typename T>
@@ -162,20 +181,118 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
   const T* invd = jac ? ML->inv_diag<T>() : nullptr;
   const bool spmv_left = ML && ML->kind == spk_precond_s::kSpmv;
   const int* done = &S->done;
+  // the one-kernel hand-off needs few partials: every block of the next
+  // kernel reads all of them
+  const bool handoff = right_id && (jac || spmv_left) &&
+                       spmv_partials_count(*A) <= kProMaxPartials &&
+                       (!spmv_left || spmv_partials_count(*ML->M) <= kProMaxPartials) &&
+                       vec_grid(n) <= kProMaxPartials;
+  double* part1 = nullptr;
+  double* part2 = nullptr;
+  double* part3 = nullptr;
+  if (handoff) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    part1 = (double*)ws.get(kWsCg + 7, pb);
+    part2 = (double*)ws.get(kWsCg + 8, pb);
+    part3 = (double*)ws.get(kWsCg + 9, pb);
+  }
 
   auto issue = [&](int64_t k) {
     if (k % cfg->restart == 0) {  // reorthogonalize(), :216-230
       if (jac) {
-        launch_spmv(*A, (const T*)x, EpiCgRestartJacobi<T>{b, invd, r, p, S, done}, s);
+        launch_spmv(*A, (const T*)x,
+                    EpiCgRestartJacobi<T>{b, invd, r, p, S, done, (int)(k & 1)}, s);
       } else {
         launch_spmv(*A, (const T*)x, EpiCgResidual<T>{b, r, done}, s);
         apply_precond(ML, n, (const T*)r, z);
+        const int par = (int)(k & 1);
         launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
           T zi = z[i];
           p[i] = zi;
           acc[0] += (double)r[i] * (double)zi;
-        }, [=] __device__(const double* t) { S->rz = t[0]; }, done);
+        }, [=] __device__(const double* t) {
+          S->rz = t[0];
+          S->rzb[par] = t[0];
+        }, done);
       }
+    }
+    if (handoff) {
+      // One-kernel hand-off: no finisher kernels.  K1 q = A p (partials
+      // <p,q>); K2 sums them itself -> alpha, r -= alpha q, partials;
+      // [K3 z = M r, partials <r,z>]; last pass sums the partials ->
+      // |r|, convergence, beta; x += alpha p; p = z + beta p.
+      const int par = (int)(k & 1);
+      const int64_t step = k + 1;
+      const int c1 = launch_spmv_partials(*A, (const T*)p, EpiCgDot<T>{p, q, S, done},
+                                          part1, s);
+      auto pro_alpha = [=] __device__(const double* t, double* sc, bool b0) {
+        const double alpha = S->rzb[par] / t[0];
+        sc[0] = alpha;
+        if (b0) {
+          S->alpha = alpha;
+          S->k2_step = step;
+        }
+      };
+      // compute_new_solution's |r| and the new beta (:241-292), once per
+      // block; block 0 records them (nothing this kernel reads)
+      auto pro_beta = [=] __device__(const double* t, double* sc, bool b0) {
+        const double rn = sqrt(t[0]);
+        const bool conv = rn / S->bnorm < S->tol;
+        const double beta = t[1] / S->rzb[par];
+        sc[0] = S->alpha;
+        sc[1] = beta;
+        if (b0) {
+          S->resnorm = rn;
+          if (S->collect) S->history[S->iters] = rn;
+          S->iters += 1;
+          if (conv) {
+            S->converged = 1;
+            S->done = 1;
+          } else {
+            S->beta = beta;
+            S->rzb[par ^ 1] = t[1];
+          }
+        }
+      };
+      if (jac) {
+        const int g2 = launch_reduce_pro<1, 0, 2>(
+            n, part1, c1, (const double*)nullptr, 0, GateNotDone{done}, pro_alpha,
+            [=] __device__(int64_t i) { return Pack4<T>{r[i], q[i], invd[i], T(0)}; },
+            [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+              const T a = (T)sc[0];
+              T ri = e.a - a * e.b;
+              r[i] = ri;
+              acc[0] += (double)ri * (double)ri;
+              acc[1] += (double)ri * (double)(e.c * ri);
+            }, part2);
+        launch_reduce_pro<2, 0, 0>(
+            n, part2, g2, (const double*)nullptr, 0, GateStep{S, step}, pro_beta,
+            [=] __device__(int64_t i) { return Pack4<T>{p[i], x[i], invd[i], r[i]}; },
+            [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+              x[i] = (T)sc[0] * e.a + e.b;
+              p[i] = e.c * e.d + (T)sc[1] * e.a;
+            }, (double*)nullptr);
+      } else {  // explicit approximate inverse
+        const int g2 = launch_reduce_pro<1, 0, 1>(
+            n, part1, c1, (const double*)nullptr, 0, GateNotDone{done}, pro_alpha,
+            [=] __device__(int64_t i) { return Pack4<T>{r[i], q[i], T(0), T(0)}; },
+            [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+              const T a = (T)sc[0];
+              T ri = e.a - a * e.b;
+              r[i] = ri;
+              acc[0] += (double)ri * (double)ri;
+            }, part2);
+        const int c3 = launch_spmv_partials(*ML->M, (const T*)r,
+                                            EpiCgPrecDot<T>{r, z, S, done}, part3, s);
+        launch_reduce_pro<1, 1, 0>(
+            n, part2, g2, part3, c3, GateStep{S, step}, pro_beta,
+            [=] __device__(int64_t i) { return Pack4<T>{p[i], x[i], z[i], T(0)}; },
+            [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+              x[i] = (T)sc[0] * e.a + e.b;
+              p[i] = e.c + (T)sc[1] * e.a;
+            }, (double*)nullptr);
+      }
+      return;
     }
     // compute_step_size(), :232-239
     if (!right_id) apply_precond(MR, n, (const T*)p, hatp);

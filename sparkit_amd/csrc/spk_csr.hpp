@@ -264,6 +264,35 @@ void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
   }
 }
 
+// The reduction's block partials (A.nblocks of them, or the last panel's)
+// go to `parts` and no finisher runs: the next kernel sums them itself.
+template <typename T, typename Epi>
+int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
+                         hipStream_t s) {
+  static_assert(Epi::kReduce, "an epilogue with a sum");
+  spk_csr_s& last = A.panels.empty() ? A : *A.panels.back();
+  const T* init = nullptr;
+  if (!A.panels.empty()) {
+    T* acc = A.panel_acc<T>();
+    for (size_t p = 0; p + 1 < A.panels.size(); ++p) {
+      EpiStore<T> part;
+      part.y = acc;
+      part.done = epi.done;
+      launch_spmv_one(*A.panels[p], x, part, init, s);
+      init = acc;
+    }
+  }
+  hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(last.nblocks), dim3(kSpmvBlock),
+                     0, s, last.view<T>(), x, epi, parts, init);
+  SPK_HIP_TRY(hipGetLastError());
+  return last.nblocks;
+}
+
+// the number of partials launch_spmv_partials writes
+inline int spmv_partials_count(const spk_csr_s& A) {
+  return A.panels.empty() ? A.nblocks : A.panels.back()->nblocks;
+}
+
 template <typename T, typename Epi>
 void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
   if (A.nrows == 0) return;
