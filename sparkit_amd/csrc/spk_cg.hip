@@ -186,7 +186,7 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
   const bool handoff = right_id && (jac || spmv_left) &&
                        spmv_partials_count(*A) <= kProMaxPartials &&
                        (!spmv_left || spmv_partials_count(*ML->M) <= kProMaxPartials) &&
-                       vec_grid(n) <= kProMaxPartials;
+                       pro_grid(n) <= kProMaxPartials;
   double* part1 = nullptr;
   double* part2 = nullptr;
   double* part3 = nullptr;

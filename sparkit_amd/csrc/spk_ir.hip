@@ -52,6 +52,20 @@ struct RcgState {
   double bnorm, rnorm, rupd, rz, alpha, beta, tol, delta;
   int64_t iters, updates;
   int32_t hold, converged, pad0, pad1;
+  // one-kernel hand-off: rz by step parity, and the step whose second
+  // kernel ran (gates the third)
+  double rzb[2];
+  int64_t k2_step;
+};
+
+struct RcgGateRun {
+  const int* hold;
+  __device__ bool operator()() const { return !*hold; }
+};
+struct RcgGateStep {
+  const RcgState* st;
+  int64_t step;
+  __device__ bool operator()() const { return st->k2_step == step; }
 };
 
 struct EpiRcgDot {  // q = A p, partial <p, q>
@@ -77,9 +91,11 @@ struct EpiRcgPrec {  // z = M r (an explicit approximate inverse), <r, z>
     z[i] = s;
     return (double)r[i] * (double)s;
   }
+  int par = 0;  // parity slot for the hand-off's rz
   __device__ void finalize(double rz) const {
     st->beta = rz / st->rz;
     st->rz = rz;
+    st->rzb[par] = rz;
   }
 };
 
@@ -134,10 +150,27 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   const bool jac = is_jacobi(M), ident = is_identity(M);
   const bool spai = M && M->kind == spk_precond_s::kSpmv;
   const float* invd = jac ? M->inv_diag<float>() : nullptr;
+  // the one-kernel hand-off (every block of the next kernel sums the
+  // previous partials) when the partials are few
+  const bool handoff = (jac || ident || spai) &&
+                       spmv_partials_count(*A) <= kProMaxPartials &&
+                       pro_grid(n) <= kProMaxPartials &&
+                       (!spai || spmv_partials_count(*M->M) <= kProMaxPartials);
+  double* part1 = nullptr;
+  double* part2 = nullptr;
+  double* part3 = nullptr;
+  if (handoff) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    part1 = (double*)ws.get(kWsIr + 8, pb);
+    part2 = (double*)ws.get(kWsIr + 9, pb);
+    part3 = (double*)ws.get(kWsIr + 10, pb);
+  }
+  int64_t issued = 0;
 
   // r = (float) r64, xi = 0, z = M r, beta = <r,z> / rz, rz = <r,z>; then
   // p = z + beta p (not when the update converged)
   auto restart = [&](bool first) {
+    const int par = (int)(issued & 1);
     if (first) launch_map(n, [=] __device__(int64_t i) { p[i] = 0.0f; });
     if (jac || ident) {
       launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
@@ -150,6 +183,7 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
       }, [=] __device__(const double* t) {
         S->beta = t[0] / S->rz;
         S->rz = t[0];
+        S->rzb[par] = t[0];
       });
     } else {
       launch_map(n, [=] __device__(int64_t i) {
@@ -157,7 +191,7 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
         xi[i] = 0.0f;
       });
       if (spai) {
-        launch_spmv(*M->M, (const float*)r32, EpiRcgPrec{r32, z, S, nullptr}, s);
+        launch_spmv(*M->M, (const float*)r32, EpiRcgPrec{r32, z, S, nullptr, par}, s);
       } else {
         M->apply((const float*)r32, z);
         launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
@@ -165,6 +199,7 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
         }, [=] __device__(const double* t) {
           S->beta = t[0] / S->rz;
           S->rz = t[0];
+          S->rzb[par] = t[0];
         });
       }
     }
@@ -173,7 +208,95 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
     }, hold);
   };
 
+  // One step through the hand-off: K1 q = A p (partials <p,q>); K2 sums
+  // them itself -> alpha, r -= alpha q, partials |r|^2 (and <r,z> for a
+  // diagonal M); [K3 z = M r, partials <r,z>]; the last pass sums the
+  // partials -> update due?, beta; xi += alpha p; p = z + beta p unless an
+  // update is due (then the restart after it forms p).  Block 0 records
+  // the scalars; rz of step k lives in rzb[k & 1].
+  auto step_handoff = [&]() {
+    const int par = (int)(issued & 1);
+    const int64_t sid = issued + 1;
+    const int c1 = launch_spmv_partials(*A, (const float*)p, EpiRcgDot{p, q, S, hold},
+                                        part1, s);
+    auto pro_alpha = [=] __device__(const double* t, double* sc, bool b0) {
+      const double alpha = S->rzb[par] / t[0];
+      sc[0] = alpha;
+      if (b0) {
+        S->alpha = alpha;
+        S->k2_step = sid;
+      }
+    };
+    // rcg_due's test and the new beta, once per block; block 0 records
+    // them (nothing this kernel reads)
+    auto pro_beta = [=] __device__(const double* t, double* sc, bool b0) {
+      const double rn = sqrt(t[0]);
+      const bool due = rn <= S->delta * S->rupd || rn / S->bnorm < S->tol;
+      const double beta = t[1] / S->rzb[par];
+      sc[0] = S->alpha;
+      sc[1] = beta;
+      sc[2] = due ? 1.0 : 0.0;
+      if (b0) {
+        S->iters += 1;
+        if (due) {
+          S->hold = 1;
+          S->rz = S->rzb[par];
+        } else {
+          S->beta = beta;
+          S->rz = t[1];
+          S->rzb[par ^ 1] = t[1];
+        }
+      }
+    };
+    if (!spai) {
+      const int g2 = launch_reduce_pro<1, 0, 2>(
+          n, part1, c1, (const double*)nullptr, 0, RcgGateRun{hold}, pro_alpha,
+          [=] __device__(int64_t i) {
+            return Pack4<float>{r32[i], q[i], invd ? invd[i] : 1.0f, 0.0f};
+          },
+          [=] __device__(int64_t i, const Pack4<float>& e, double* acc, const double* sc) {
+            const float a = (float)sc[0];
+            const float ri = e.a - a * e.b;
+            r32[i] = ri;
+            acc[0] += (double)ri * (double)ri;
+            acc[1] += (double)ri * (double)(invd ? e.c * ri : ri);
+          }, part2);
+      launch_reduce_pro<2, 0, 0>(
+          n, part2, g2, (const double*)nullptr, 0, RcgGateStep{S, sid}, pro_beta,
+          [=] __device__(int64_t i) {
+            return Pack4<float>{p[i], xi[i], r32[i], invd ? invd[i] : 1.0f};
+          },
+          [=] __device__(int64_t i, const Pack4<float>& e, double*, const double* sc) {
+            xi[i] = (float)sc[0] * e.a + e.b;
+            if (sc[2] == 0.0) p[i] = (invd ? e.d * e.c : e.c) + (float)sc[1] * e.a;
+          }, (double*)nullptr);
+    } else {
+      const int g2 = launch_reduce_pro<1, 0, 1>(
+          n, part1, c1, (const double*)nullptr, 0, RcgGateRun{hold}, pro_alpha,
+          [=] __device__(int64_t i) { return Pack4<float>{r32[i], q[i], 0.0f, 0.0f}; },
+          [=] __device__(int64_t i, const Pack4<float>& e, double* acc, const double* sc) {
+            const float a = (float)sc[0];
+            const float ri = e.a - a * e.b;
+            r32[i] = ri;
+            acc[0] += (double)ri * (double)ri;
+          }, part2);
+      const int c3 = launch_spmv_partials(*M->M, (const float*)r32,
+                                          EpiRcgPrec{r32, z, S, hold}, part3, s);
+      launch_reduce_pro<1, 1, 0>(
+          n, part2, g2, part3, c3, RcgGateStep{S, sid}, pro_beta,
+          [=] __device__(int64_t i) { return Pack4<float>{p[i], xi[i], z[i], 0.0f}; },
+          [=] __device__(int64_t i, const Pack4<float>& e, double*, const double* sc) {
+            xi[i] = (float)sc[0] * e.a + e.b;
+            if (sc[2] == 0.0) p[i] = e.c + (float)sc[1] * e.a;
+          }, (double*)nullptr);
+    }
+  };
+
   auto step = [&]() {
+    if (handoff) {
+      step_handoff();
+      return;
+    }
     launch_spmv(*A, (const float*)p, EpiRcgDot{p, q, S, hold}, s);
     if (jac || ident) {
       launch_reduce<2>(n, [=] __device__(int64_t i, double* acc) {
@@ -218,7 +341,7 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   };
 
   restart(true);
-  int64_t issued = 0, k = 0;
+  int64_t k = 0;
   if (history) history[0] = rnorm / bnorm;
   for (;;) {
     // FP32 steps until an update is due or the budget is spent

@@ -180,12 +180,27 @@ __global__ __launch_bounds__(kVecBlock) void reduce_pro_kernel(
   }
 }
 
+// Blocks per CU of a prologue kernel: every block re-reads all partials
+// of the previous kernel, so the grid (and with it the number of partials
+// the next prologue reads) is kept smaller than vec_grid's.
+#ifndef SPK_PRO_BLOCKS_PER_CU
+#define SPK_PRO_BLOCKS_PER_CU 4
+#endif
+inline int pro_grid(int64_t n) {
+  int64_t g = (n + kVecBlock - 1) / kVecBlock;
+  int64_t cap = (int64_t)ctx().num_cus * SPK_PRO_BLOCKS_PER_CU;
+  if (cap < 64) cap = 64;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
 // Returns the grid (= the number of WO-wide partials written to parts_out).
 template <int WI, int WJ, int WO, typename Gate, typename Pro, typename Ld, typename F>
 int launch_reduce_pro(int64_t n, const double* parts_in, int count_in,
                       const double* parts_in2, int count_in2, Gate gate, Pro pro,
                       Ld ld, F f, double* parts_out) {
-  const int g = vec_grid(n);
+  const int g = pro_grid(n);
   hipLaunchKernelGGL((reduce_pro_kernel<WI, WJ, WO, Gate, Pro, Ld, F>), dim3(g),
                      dim3(kVecBlock), 0, stream(), n, parts_in, count_in,
                      parts_in2, count_in2, gate, pro, ld, f, parts_out);
