@@ -34,6 +34,7 @@
 #include <map>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "spk_csr.hpp"
@@ -128,7 +129,35 @@ struct BatchArgs {
   double tol, inner_tol;
   int max_outer, max_inner;
   int lds_rows;
+  // sliced-ELL copy of the inner-precision matrix (ell_w slots per row,
+  // slot-major within each system: slot k of local row i of the system
+  // starting at global row r0 (n rows) at r0 * ell_w + k * n + i; short
+  // rows padded with (own row, 0)), used when ell_w > 0
+  const int32_t* __restrict__ ell_ci;
+  const void* __restrict__ ell_v;
+  int64_t ell_n;
+  int ell_w;
 };
+
+// sliced-ELL widths compiled (rows are padded to the next one; wider rows
+// keep the CSR loop); rows of the inner loop per thread and pass (their
+// loads are all issued before the first use)
+constexpr int kEllMax = 9;
+inline int ell_width_for(int w) {
+  return w <= 3 ? 3 : w <= 5 ? 5 : w <= 7 ? 7 : w <= 9 ? 9 : 0;
+}
+constexpr int kBatchRows = 2;
+
+// element j of a per-system base through a 32-bit byte offset (systems
+// hold fewer than 2^29 rows)
+template <typename T>
+__device__ __forceinline__ T ld32(const T* base, int j) {
+  return *(const T*)((const char*)base + (uint32_t)j * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* base, int j, T v) {
+  *(T*)((char*)base + (uint32_t)j * (uint32_t)sizeof(T)) = v;
+}
 
 template <int BLOCK, int W>
 __device__ __forceinline__ void wg_sum(double (&v)[W], double* lds, int& phase) {
@@ -152,7 +181,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[W], double* lds, int& phase) 
   phase ^= 1;
 }
 
-template <int BLOCK, typename T>
+template <int BLOCK, typename T, int EW>
 __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
   constexpr int NW = BLOCK / 64;
   __shared__ double red[2 * NW * 2];
@@ -224,76 +253,165 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
       wg_sum<BLOCK, 1>(rz, red, phase);
       double rzv = rz[0];
       double rupd = rnorm;  // |r| at the last FP64 update
-      for (int it = 0; it < a.max_inner; ++it) {
-        __syncthreads();  // p complete
-        double pq[1] = {0.0};
-        for (int i = tid; i < n; i += BLOCK) {
-          const int64_t g = r0 + i;
-          T s = (T)0;
-          for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) s += vin[q] * pget(a.ci[q]);
-          Q[g] = s;
-          pq[0] += (double)pget(g) * (double)s;
-        }
-        wg_sum<BLOCK, 1>(pq, red, phase);
-        if (!(pq[0] > 0.0)) break;  // breakdown
-        const T alpha = (T)(rzv / pq[0]);
-        double st[2] = {0.0, 0.0};
-        for (int i = tid; i < n; i += BLOCK) {
-          const int64_t g = r0 + i;
-          Dv[g] = Dv[g] + alpha * pget(g);
-          const T ri = R[g] - alpha * Q[g];
-          R[g] = ri;
-          const T zi = invd[g] * ri;
-          st[0] += (double)ri * (double)zi;
-          st[1] += (double)ri * (double)ri;
-        }
-        wg_sum<BLOCK, 2>(st, red, phase);
-        ++inner_total;
-        const double rn = sqrt(st[1]);
-        if (rn < a.inner_tol * rupd || rn < a.tol * bnorm) {
-          // reliable update: x += d, true residual in FP64
-          for (int i = tid; i < n; i += BLOCK) {
-            const int64_t g = r0 + i;
-            a.x[g] += (double)Dv[g];
-            Dv[g] = (T)0;
+      // The three passes take kBatchRows rows per thread at a time (rows
+      // i, i + BLOCK, ...), issuing every load of those rows before the
+      // first use; the sums run over the rows in the same order as one row
+      // at a time.
+      constexpr int U = kBatchRows;
+      // per-system bases (uniform) and 32-bit local row offsets: the
+      // loads take a scalar base and one 32-bit vector offset
+      T* __restrict__ Rs = R + r0;
+      T* __restrict__ Ds = Dv + r0;
+      T* __restrict__ Ps = P + r0;
+      T* __restrict__ Qs = Q + r0;
+      const T* __restrict__ Is = invd + r0;
+      // the inner loop is compiled twice: p in LDS (ds loads) or in global
+      // memory (global loads), never a generic pointer
+      auto inner = [&](auto lds_tag) {
+        constexpr bool kL = decltype(lds_tag)::value;
+        auto pl = [&](int j) -> T {
+          if constexpr (kL) return lds_p[j];
+          else return ld32((const T*)Ps, j);
+        };
+        for (int it = 0; it < a.max_inner; ++it) {
+          __syncthreads();  // p complete
+          double pq[1] = {0.0};
+          if constexpr (EW > 0) {
+            const int rb = (int)r0;
+            const int32_t* __restrict__ cs = a.ell_ci + r0 * EW;
+            const T* __restrict__ vs = (const T*)a.ell_v + r0 * EW;
+            for (int i = tid; i < n; i += U * BLOCK) {
+              int32_t c[U][EW];
+              T v[U][EW];
+  #pragma unroll
+              for (int u = 0; u < U; ++u) {
+                int o = i + u * BLOCK;
+                if (o < n) {
+  #pragma unroll
+                  for (int k = 0; k < EW; ++k) {
+                    c[u][k] = ld32(cs, o) - rb;
+                    v[u][k] = ld32(vs, o);
+                    o += n;
+                  }
+                }
+              }
+  #pragma unroll
+              for (int u = 0; u < U; ++u) {
+                const int j = i + u * BLOCK;
+                if (j < n) {
+                  T sum = (T)0;
+  #pragma unroll
+                  for (int k = 0; k < EW; ++k) sum += v[u][k] * pl(c[u][k]);
+                  st32(Qs, j, sum);
+                  pq[0] += (double)pl(j) * (double)sum;
+                }
+              }
+            }
+          } else {
+            for (int i = tid; i < n; i += BLOCK) {
+              const int64_t g = r0 + i;
+              T sum = (T)0;
+              for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) sum += vin[q] * pl(a.ci[q] - (int)r0);
+              Q[g] = sum;
+              pq[0] += (double)pl(i) * (double)sum;
+            }
           }
-          __syncthreads();  // x complete
-          double rr[1] = {0.0};
-          for (int i = tid; i < n; i += BLOCK) {
-            const int64_t g = r0 + i;
-            double s = 0.0;
-            for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) s += a.v64[q] * a.x[a.ci[q]];
-            const double ri = a.b[g] - s;
-            a.r64[g] = ri;
-            rr[0] += ri * ri;
+          wg_sum<BLOCK, 1>(pq, red, phase);
+          if (!(pq[0] > 0.0)) break;  // breakdown
+          const T alpha = (T)(rzv / pq[0]);
+          double st[2] = {0.0, 0.0};
+          for (int i = tid; i < n; i += U * BLOCK) {
+            T dv[U], pv[U], rv[U], qv[U], iv[U];
+  #pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int j = i + u * BLOCK;
+              if (j < n) {
+                dv[u] = ld32(Ds, j);
+                pv[u] = pl(j);
+                rv[u] = ld32(Rs, j);
+                qv[u] = ld32(Qs, j);
+                iv[u] = ld32(Is, j);
+              }
+            }
+  #pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int j = i + u * BLOCK;
+              if (j < n) {
+                st32(Ds, j, dv[u] + alpha * pv[u]);
+                const T ri = rv[u] - alpha * qv[u];
+                st32(Rs, j, ri);
+                const T zi = iv[u] * ri;
+                st[0] += (double)ri * (double)zi;
+                st[1] += (double)ri * (double)ri;
+              }
+            }
           }
-          wg_sum<BLOCK, 1>(rr, red, phase);
-          rnorm = sqrt(rr[0]);
-          rel = rnorm / bnorm;
-          ++outer;
-          if (rel < a.tol) {
-            conv = 1;
-            break;
+          wg_sum<BLOCK, 2>(st, red, phase);
+          ++inner_total;
+          const double rn = sqrt(st[1]);
+          if (rn < a.inner_tol * rupd || rn < a.tol * bnorm) {
+            // reliable update: x += d, true residual in FP64
+            for (int i = tid; i < n; i += BLOCK) {
+              const int64_t g = r0 + i;
+              a.x[g] += (double)Dv[g];
+              Dv[g] = (T)0;
+            }
+            __syncthreads();  // x complete
+            double rr[1] = {0.0};
+            for (int i = tid; i < n; i += BLOCK) {
+              const int64_t g = r0 + i;
+              double s = 0.0;
+              for (int q = a.rp[g]; q < a.rp[g + 1]; ++q) s += a.v64[q] * a.x[a.ci[q]];
+              const double ri = a.b[g] - s;
+              a.r64[g] = ri;
+              rr[0] += ri * ri;
+            }
+            wg_sum<BLOCK, 1>(rr, red, phase);
+            rnorm = sqrt(rr[0]);
+            rel = rnorm / bnorm;
+            ++outer;
+            if (rel < a.tol) {
+              conv = 1;
+              break;
+            }
+            if (outer >= a.max_outer) break;
+            rupd = rnorm;
+            double rzn[1] = {0.0};
+            for (int i = tid; i < n; i += BLOCK) {
+              const int64_t g = r0 + i;
+              const T ri = (T)a.r64[g];
+              R[g] = ri;
+              rzn[0] += (double)ri * (double)(invd[g] * ri);
+            }
+            wg_sum<BLOCK, 1>(rzn, red, phase);
+            st[0] = rzn[0];
           }
-          if (outer >= a.max_outer) break;
-          rupd = rnorm;
-          double rzn[1] = {0.0};
-          for (int i = tid; i < n; i += BLOCK) {
-            const int64_t g = r0 + i;
-            const T ri = (T)a.r64[g];
-            R[g] = ri;
-            rzn[0] += (double)ri * (double)(invd[g] * ri);
+          const T beta = (T)(st[0] / rzv);
+          rzv = st[0];
+          for (int i = tid; i < n; i += U * BLOCK) {
+            T pv[U], rv[U], iv[U];
+  #pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int j = i + u * BLOCK;
+              if (j < n) {
+                iv[u] = ld32(Is, j);
+                rv[u] = ld32(Rs, j);
+                pv[u] = pl(j);
+              }
+            }
+  #pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int j = i + u * BLOCK;
+              if (j < n) {
+                const T pn = iv[u] * rv[u] + beta * pv[u];
+                if constexpr (kL) lds_p[j] = pn; else st32(Ps, j, pn);
+              }
+            }
           }
-          wg_sum<BLOCK, 1>(rzn, red, phase);
-          st[0] = rzn[0];
         }
-        const T beta = (T)(st[0] / rzv);
-        rzv = st[0];
-        for (int i = tid; i < n; i += BLOCK) {
-          const int64_t g = r0 + i;
-          pset(g, invd[g] * R[g] + beta * pget(g));
-        }
-      }
+      };
+      if (in_lds) inner(std::true_type{});
+      else inner(std::false_type{});
       if (!conv) {  // budget spent: keep the correction made so far
         for (int i = tid; i < n; i += BLOCK) a.x[r0 + i] += (double)Dv[r0 + i];
         __syncthreads();
@@ -339,6 +457,48 @@ __global__ void batch_cast_kernel(int64_t n, const double* __restrict__ in,
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+__global__ void batch_maxlen_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                    int* __restrict__ out) {
+  int m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, rp[i + 1] - rp[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// slot k of global row g (local row g - r0 of the system [r0, r1)): the
+// row's k-th entry in storage order, or (g, 0); sys_of[g] is found by a
+// binary search of the row offsets
+template <typename T>
+__global__ void batch_ell_kernel(int64_t n, int w, const int64_t* __restrict__ sys_row,
+                                 int K, const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci,
+                                 const double* __restrict__ v,
+                                 int32_t* __restrict__ eci, T* __restrict__ ev) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = K;  // largest k with sys_row[k] <= g
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sys_row[mid] <= g) lo = mid; else hi = mid;
+    }
+    const int64_t r0 = sys_row[lo], len_sys = sys_row[lo + 1] - r0;
+    const int b = rp[g], len = rp[g + 1] - b;
+    for (int k = 0; k < w; ++k) {
+      const int64_t o = r0 * w + (int64_t)k * len_sys + (g - r0);
+      if (k < len) {
+        if (eci) eci[o] = ci[b + k];
+        ev[o] = (T)v[b + k];
+      } else {
+        if (eci) eci[o] = (int32_t)g;
+        ev[o] = (T)0;
+      }
+    }
+  }
+}
+
 }  // namespace
 }  // namespace spk
 
@@ -355,6 +515,12 @@ struct spk_batch_s {
   DevBuf<float> r32, d32, p32, q32, invd32;
   DevBuf<double> rI, dI, pI, qI, invd64;  // FP64 inner vectors, made on first use
   DevBuf<int> counter;
+  // sliced-ELL copy of the matrix for the inner loop (rows of at most
+  // kEllMax entries); the FP64 values are made on first use
+  int ell_w = 0;
+  DevBuf<int32_t> ell_ci;
+  DevBuf<float> ell_v32;
+  DevBuf<double> ell_v64;
   DevBuf<int32_t> iters, outer, conv;
   DevBuf<double> relres;
   spk_batch_tuning tuning{256, 2, 0, 0};
@@ -417,15 +583,48 @@ struct spk_batch_s {
     SPK_HIP_TRY(hipStreamSynchronize(s));
     SPK_REQUIRE(!hb, SPK_INVALID_ARGUMENT, "zero diagonal element in Jacobi");
     (void)A->vals32();
+    // the sliced-ELL copy when every row is short
+    DevBuf<int> mx(1);
+    SPK_HIP_TRY(hipMemsetAsync(mx.p, 0, 4, s));
+    hipLaunchKernelGGL(batch_maxlen_kernel, dim3(std::min<unsigned>(nblk(total), 4096)),
+                       dim3(256), 0, s, total, (const int32_t*)A->rp.p, mx.p);
+    SPK_HIP_TRY(hipGetLastError());
+    int w = 0;
+    SPK_HIP_TRY(hipMemcpyAsync(&w, mx.p, 4, hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    if (w >= 1 && ell_width_for(w) > 0 &&
+        (int64_t)total * ell_width_for(w) < ((int64_t)1 << 31)) {
+      w = ell_width_for(w);
+      ell_w = w;
+      ell_ci.alloc(n * (size_t)w);
+      ell_v32.alloc(n * (size_t)w);
+      hipLaunchKernelGGL(batch_ell_kernel<float>, dim3(std::min<unsigned>(nblk(total), 16384)),
+                         dim3(256), 0, s, total, w, (const int64_t*)sys_row.p, (int)K,
+                         (const int32_t*)A->rp.p,
+                         (const int32_t*)A->ci.p, (const double*)A->v64.p, ell_ci.p,
+                         ell_v32.p);
+      SPK_HIP_TRY(hipGetLastError());
+    }
   }
 
   void ensure_f64() {
-    if (rI.n) return;
-    const size_t n = (size_t)total;
-    rI.alloc(n);
-    dI.alloc(n);
-    pI.alloc(n);
-    qI.alloc(n);
+    if (!rI.n) {
+      const size_t n = (size_t)total;
+      rI.alloc(n);
+      dI.alloc(n);
+      pI.alloc(n);
+      qI.alloc(n);
+    }
+    if (ell_w > 0 && !ell_v64.n) {
+      ell_v64.alloc((size_t)total * (size_t)ell_w);
+      hipLaunchKernelGGL(batch_ell_kernel<double>,
+                         dim3(std::min<unsigned>(nblk(total), 16384)), dim3(256), 0,
+                         stream(), total, ell_w, (const int64_t*)sys_row.p, (int)K,
+                         (const int32_t*)A->rp.p,
+                         (const int32_t*)A->ci.p, (const double*)A->v64.p,
+                         (int32_t*)nullptr, ell_v64.p);
+      SPK_HIP_TRY(hipGetLastError());
+    }
   }
 
   BatchArgs args(const spk_batch_config& c, const double* b, double* x,
@@ -456,6 +655,10 @@ struct spk_batch_s {
       a.q = qI.p;
       a.invd = invd64.p;
     }
+    a.ell_w = ell_w;
+    a.ell_n = total;
+    a.ell_ci = ell_ci.p;
+    a.ell_v = prec == SPK_F32 ? (const void*)ell_v32.p : (const void*)ell_v64.p;
     a.counter = counter.p;
     a.iters = iters.p;
     a.outer = outer.p;
@@ -483,8 +686,18 @@ struct spk_batch_s {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(nblk(std::max<int64_t>(reset_rows, 1))),
                        dim3(256), 0, s, reset_rows, x, counter.p);
     const int grid = std::max(1, t.wg_per_cu * ctx().num_cus);
-#define SPK_BATCH_LAUNCH(BS, TT)                                                \
-  hipLaunchKernelGGL((batch_pcg_kernel<BS, TT>), dim3(grid), dim3(BS), dyn, s, a)
+#define SPK_BATCH_EW(BS, TT, EW) \
+  hipLaunchKernelGGL((batch_pcg_kernel<BS, TT, EW>), dim3(grid), dim3(BS), dyn, s, a)
+#define SPK_BATCH_LAUNCH(BS, TT)                     \
+  do {                                               \
+    switch (a.ell_w) {                               \
+      case 3: SPK_BATCH_EW(BS, TT, 3); break;        \
+      case 5: SPK_BATCH_EW(BS, TT, 5); break;        \
+      case 7: SPK_BATCH_EW(BS, TT, 7); break;        \
+      case 9: SPK_BATCH_EW(BS, TT, 9); break;        \
+      default: SPK_BATCH_EW(BS, TT, 0); break;       \
+    }                                                \
+  } while (0)
     if (prec == SPK_F32) {
       if (t.block == 1024) SPK_BATCH_LAUNCH(1024, float);
       else if (t.block == 512) SPK_BATCH_LAUNCH(512, float);
@@ -495,6 +708,7 @@ struct spk_batch_s {
       else SPK_BATCH_LAUNCH(256, double);
     }
 #undef SPK_BATCH_LAUNCH
+#undef SPK_BATCH_EW
     SPK_HIP_TRY(hipGetLastError());
   }
 };

@@ -119,3 +119,64 @@ def test_from_blocks_and_errors(O, spk):
         spk.BatchedSystems.from_blocks(A, [0, n1 - 1, n1 + n2])
     with pytest.raises(spk.InvalidArgument):
         B.tuning = (300, 1, 0)
+
+
+def _arrow(O, n):
+    """Arrowhead SPD matrix: row 0 holds all n entries (wider than any
+    sliced-ELL width), the rest tridiagonal plus the first column."""
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                v = 3.0 * n
+            elif i == 0 or j == 0 or abs(i - j) == 1:
+                v = -1.0
+            else:
+                continue
+            rows.append(i), cols.append(j), vals.append(v)
+    return O.make_matrix(n, n, rows, cols, vals)
+
+
+@pytest.mark.parametrize("kind", ["w3", "w5", "w7", "w9", "csr"])
+def test_every_row_width(O, spk, kind):
+    """Batches whose widest row takes each compiled sliced-ELL width (rows
+    padded to it), and one with a 12-entry row that keeps the CSR loop:
+    per-system FP64 iteration counts match the oracle's Jacobi-PCG, FP32
+    reaches the FP64 true-residual tolerance."""
+    mk = {"w3": lambda: [O.tridiagonal_matrix(40, -1.0, 2.5, -1.0),
+                         O.tridiagonal_matrix(17, -1.0, 3.0, -1.0)],
+          "w5": lambda: [O.poisson_2d(9, 7, 0.1), O.tridiagonal_matrix(30, -1.0, 2.2, -1.0)],
+          "w7": lambda: [O.poisson_3d(6, 5, 4), O.poisson_2d(8, 8, 0.05)],
+          "w9": lambda: [O.aniso_q1_2d(12, 5, 1.5), O.poisson_3d(5, 5, 5)],
+          "csr": lambda: [_arrow(O, 12), O.poisson_2d(6, 6, 0.1)]}[kind]
+    blocks = mk()
+    offs, rps, cis, vs, nz = [0], [np.zeros(1, np.int32)], [], [], 0
+    for S in blocks:
+        r0 = offs[-1]
+        rps.append((S.row_ptr[1:] + nz).astype(np.int32))
+        cis.append((S.col_ind + r0).astype(np.int32))
+        vs.append(S.vals)
+        offs.append(r0 + S.nrows)
+        nz += S.nnz
+    n = offs[-1]
+    A = spk.CsrMatrix.from_host(n, n, np.concatenate(rps), np.concatenate(cis),
+                                np.concatenate(vs))
+    B = spk.BatchedSystems.from_blocks(A, offs)
+    bh = O.uniform_pm1(9, n)
+    for prec in ("f64", "f32"):
+        x = spk.DeviceVector.zeros(n)
+        if prec == "f64":
+            cfg = spk.BatchConfig(1e-9, 1e-9, 1, 3000, spk.F64)
+        else:
+            cfg = spk.BatchConfig(1e-9, 1e-3, 40, 3000, spk.F32)
+        s = B.solve(cfg, spk.DeviceVector.from_numpy(bh), x)
+        assert s["converged"] == len(blocks), (kind, prec)
+        it = B.results()[0]
+        xh = x.numpy()
+        for k, S in enumerate(blocks):
+            bk, xk = bh[offs[k]:offs[k + 1]], xh[offs[k]:offs[k + 1]]
+            assert O.true_relres(S, bk, xk) < 1e-9, (kind, prec, k)
+            if prec == "f64":
+                _, sr = O.cg(S, bk, tol=1e-9, restart_iterations=3000,
+                             max_iterations=3000, left="jacobi")
+                assert abs(it[k] - sr.computed_iterations) <= 2, (kind, k)
