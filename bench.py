@@ -402,6 +402,79 @@ def mmio_section(spk, grid=96):
             "round_trip_exact": same}
 
 
+def frobenius_optimal_spai(n, rp, ci, v):
+    """The SPAI the network's loss aims at, computed exactly: row i of M
+    minimises ||e_i^T - m_i^T A||_2 over the pattern J of row i of A, i.e.
+    (A A^T)[J, J] m = A[J, i] -- one small SPD system per row, solved in
+    batches (rows padded to the widest with identity rows)."""
+    import scipy.sparse as sps
+    A = sps.csr_matrix((v, ci, rp), shape=(n, n))
+    B = (A @ A.T).tocsr()
+    lens = np.diff(rp)
+    W = int(lens.max())
+    J = -np.ones((n, W), np.int64)
+    pos = np.arange(len(ci)) - np.repeat(rp[:-1], lens)
+    J[np.repeat(np.arange(n), lens), pos] = ci
+    ok = J >= 0
+    Jc = np.where(ok, J, 0)
+    G = np.zeros((n, W, W))
+    for a in range(W):
+        for c in range(W):
+            g = np.asarray(B[Jc[:, a], Jc[:, c]]).ravel()
+            G[:, a, c] = np.where(ok[:, a] & ok[:, c], g, 0.0)
+        G[:, a, a] = np.where(ok[:, a], G[:, a, a], 1.0)
+    rhs = np.asarray(A[Jc.ravel(), np.repeat(np.arange(n), W)]).reshape(n, W)
+    m = np.linalg.solve(G, np.where(ok, rhs, 0.0)[:, :, None])[:, :, 0]
+    return m[ok]
+
+
+def spai_ceiling_section(spk, n_side=1024, train_side=256, train_count=512,
+                         epochs=24, lr=2e-3, seed=4242):
+    """How close the learned SPAI is to the best SPAI on A's pattern: the
+    loss (1/n)||I - M A||_F^2 of Jacobi, of the trained network and of the
+    Frobenius-optimal M (computed exactly on the host), and FP64 PCG to
+    1e-8 with the symmetric part of each."""
+    import scipy.sparse as sps
+    A = spk.aniso_q1_2d(n_side, seed, 1.5)
+    n = A.nrows
+    rp, ci, v = A.download()
+    rp, ci = rp.astype(np.int64), ci.astype(np.int64)
+    mv, t_exact = _timed(lambda: frobenius_optimal_spai(n, rp, ci, v))
+    mats = [spk.aniso_q1_2d(train_side, s, 1.5) for s in range(train_count)]
+    model = spk.SpaiModel(1)
+    model.train(mats, epochs=epochs, learning_rate=lr, shuffle_seed=7)
+    spk.synchronize()
+    del mats
+    Ml = model.predict(A)
+    diag = np.repeat(np.arange(n), np.diff(rp)) == ci
+    jac = np.zeros_like(v)
+    jac[diag] = 1.0 / v[diag]
+    Ad = sps.csr_matrix((v, ci, rp), shape=(n, n))
+
+    def loss(m):
+        R = sps.identity(n, format="csr") - sps.csr_matrix((m, ci, rp), shape=(n, n)) @ Ad
+        return round(float(R.multiply(R).sum() / n), 4)
+
+    res = {"input": f"2D anisotropic Q1 diffusion {n_side}^2", "rows": n,
+           "exact_spai_host_s": round(t_exact / 1e3, 1),
+           "loss": {"jacobi": loss(jac), "learned": loss(Ml.download()[2]),
+                    "frobenius_optimal": loss(mv)}}
+    Me = spk.CsrMatrix.from_host(n, n, rp.astype(np.int32), ci.astype(np.int32), mv)
+    b = spk.uniform_pm1(42, n)
+    cg = spk.CGConfig(1e-8, 50000, 50000)
+    for name, M in (("jacobi", spk.jacobi(A)),
+                    ("learned_sym", spk.spai(Ml.symmetric_part())),
+                    ("frobenius_optimal_sym", spk.spai(Me.symmetric_part()))):
+        x = spk.DeviceVector.zeros(n)
+        s, ms = _timed(lambda: spk.conjugate_gradient(b, x, cg, A, M))
+        res["pcg_" + name] = {"iterations": s.computed_iterations,
+                              "converged": bool(s.converged), "ms": round(ms, 1),
+                              "true_relres": spk.true_relres(A, b, x)}
+        log(f"SPAI ceiling: PCG {name}: {res['pcg_' + name]}")
+    log(f"SPAI ceiling: loss {res['loss']}")
+    return res
+
+
 def rcm_section(spk, grid=128, seed=5):
     """Reverse Cuthill-McKee on the device: the 3D Poisson grid^3 under a
     seeded random relabelling (a scrambled band), reordered by RCM and
@@ -567,8 +640,9 @@ def main():
                          "comparisons only), "
                          "batched (batched PCG), krylov (BiCGSTAB, CGS, "
                          "BiCG, QMR, CG, MINRES), mmio (Matrix Market "
-                         "round trip), rcm (device reverse Cuthill-McKee); "
-                         "'none' for neither")
+                         "round trip), rcm (device reverse Cuthill-McKee), "
+                         "spai-ceiling (learned vs Frobenius-optimal SPAI; "
+                         "not in the default list); 'none' for neither")
     ap.add_argument("--spai-epochs", type=int, default=24)
     ap.add_argument("--spai-train", type=int, default=512)
     ap.add_argument("--spai-side", type=int, default=1024)
@@ -732,6 +806,10 @@ def main():
         section("matrix_market", lambda: mmio_section(spk))
     if "rcm" in secs:
         section("rcm", lambda: rcm_section(spk))
+    if "spai-ceiling" in secs:
+        section("spai_ceiling", lambda: spai_ceiling_section(
+            spk, n_side=args.spai_side, train_count=args.spai_train,
+            epochs=args.spai_epochs))
     if not args.no_cpu and "a" in args.inputs:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)
