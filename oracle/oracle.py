@@ -460,7 +460,9 @@ def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
     x = np.zeros(n, dtype) if x0 is None else np.array(x0, dtype=dtype)
     cfg = (ConfigF if sfx == "f" else ConfigD)(tol, restart, max_iterations)
     summ = (SummaryF if sfx == "f" else SummaryD)()
-    hist = np.zeros(max(max_iterations, 1), dtype)
+    # two-matvec steps (BiCGSTAB, CGS, BiCG, QMR) can run one matvec past an
+    # odd budget, one history entry per matvec
+    hist = np.zeros(max(max_iterations, 1) + 2, dtype)
     a_op = _Op(("csr", A), A, dtype)
     ml = _Op(left, A, dtype)
     if name in ("bicg", "qmr"):

@@ -20,7 +20,42 @@ struct BiState {
   int64_t half_step;  // step index whose half-step test converged, else -1
   int32_t done, converged, breakdown, collect;
   double* history;
+  int64_t k2_step;  // one-kernel hand-off: the step whose s pass ran
 };
+
+struct BiGateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+struct BiGateStep {
+  const BiState* st;
+  int64_t step;
+  __device__ bool operator()() const { return st->k2_step == step; }
+};
+
+template <typename T>
+struct EpiBiVParts {  // v = A p^, partial <r^, v> (summed by the next kernel)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ rh;
+  T* __restrict__ v;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    v[i] = s;
+    return (double)rh[i] * (double)s;
+  }
+  __device__ void finalize(double) const {}
+};
+
+// the |r| test that closes a step (bicgstab.hpp:310-324), thread 0 of block 0
+__device__ inline void bi_r_test(BiState* S, double rr) {
+  const double rn = sqrt(rr);
+  S->resnorm = rn;
+  if (S->collect) S->history[S->iters - 1] = rn;
+  if (rn / S->bnorm < S->tol) {
+    S->converged = 1;
+    S->done = 1;
+  }
+}
 
 template <typename T>
 struct EpiBiV {  // v = A p^, partial <r^, v>; last block: alpha
@@ -100,7 +135,158 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     MR->apply((const T*)tmp, outv);
     return outv;
   };
+  // One-kernel hand-off (the SpMV has few row blocks): no finisher
+  // kernels.  Each reduction's partials are summed by every block of the
+  // next kernel, which derives the scalar it needs; block 0 records state
+  // that no block of the same kernel reads.  Per step:
+  //   K1  prologue: the previous step's |r| test, rho_new = <r^, r>, beta;
+  //       p = r + beta (p - omega v)
+  //   S1  v = A p^, partials <r^, v>
+  //   K2  prologue: alpha; s = r - alpha v, partials |s|^2
+  //   S2  t = A s^
+  //   K4  prologue: the half-step |s| test -- on exit x += alpha p^ --;
+  //       otherwise partials <t,t>, <t,s>  (gated on K2's step stamp, as
+  //       block 0 may set done)
+  //   K5  prologue: omega; x += alpha p^ + omega s^, r = s - omega t,
+  //       partials |r|^2, <r^, r> for the next K1
+  // A step's |r| test runs in the next step's K1, or in a tail kernel
+  // after the last step.
+  const bool handoff = spmv_partials_count(*A) <= kProMaxPartials &&
+                       pro_grid(n) <= kProMaxPartials;
+  double* pq[4] = {nullptr, nullptr, nullptr, nullptr};
+  int c_v = 0, c_s = 0, c_t = 0, c_r = 0;
+  if (handoff) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    for (int i = 0; i < 4; ++i) pq[i] = (double*)ws.get(kWsBicgstab + 11 + i, pb);
+    // <r^, r> and |r|^2 of r0 for the first K1 (r^ = r)
+    c_r = launch_reduce_pro<1, 0, 2>(
+        n, pq[3], 0, (const double*)nullptr, 0, BiGateNotDone{done},
+        [=] __device__(const double*, double*, bool) {},
+        [=] __device__(int64_t i) { return Pack4<T>{r[i], T(0), T(0), T(0)}; },
+        [=] __device__(int64_t, const Pack4<T>& e, double* acc, const double*) {
+          const double ri = (double)e.a;
+          acc[0] += ri * ri;
+          acc[1] += ri * ri;
+        }, pq[3]);
+  }
+  auto issue_handoff = [&](int64_t k) {
+    const int64_t sid = k + 1;
+    // K1 (:203-206, :232-238; the previous step's :310-324)
+    const bool test_r = k > 0;
+    launch_reduce_pro<2, 0, 0>(
+        n, pq[3], c_r, (const double*)nullptr, 0, BiGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double rn2 = t[0], rho_new = t[1];
+          bool stop = false;
+          if (test_r && sqrt(rn2) / S->bnorm < S->tol) stop = true;
+          const double beta = (rho_new / S->rho) * (S->alpha / S->omega);
+          sc[0] = beta;
+          sc[1] = S->omega;
+          sc[2] = (stop || rho_new == 0.0) ? 1.0 : 0.0;
+          if (b0) {
+            if (test_r) bi_r_test(S, rn2);
+            if (!stop) {
+              S->rho_new = rho_new;
+              if (rho_new == 0.0) {
+                S->breakdown = 1;
+                S->done = 1;
+              } else {
+                S->beta = beta;
+              }
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{r[i], p[i], v[i], T(0)}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+          if (sc[2] != 0.0) return;
+          p[i] = (T)((double)e.a + sc[0] * ((double)e.b - sc[1] * (double)e.c));
+        }, (double*)nullptr);
+    // S1 (:240-250)
+    const T* php = compose(p, ph);
+    c_v = launch_spmv_partials(*A, php, EpiBiVParts<T>{rh, v, done}, pq[0], s);
+    // K2 (:211, :252-258)
+    c_s = launch_reduce_pro<1, 0, 1>(
+        n, pq[0], c_v, (const double*)nullptr, 0, BiGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double alpha = S->rho_new / t[0];
+          sc[0] = alpha;
+          if (b0) {
+            S->iters += 1;
+            S->alpha = alpha;
+            S->k2_step = sid;
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{r[i], v[i], T(0), T(0)}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+          const T si = (T)((double)e.a - sc[0] * (double)e.b);
+          sv[i] = si;
+          acc[0] += (double)si * (double)si;
+        }, pq[1]);
+    // S2 (:280-290)
+    const T* shp = compose(sv, sh);
+    launch_spmv(*A, shp, EpiStore<T>{t, done}, s);
+    // K4 (:259-278, :292-297)
+    c_t = launch_reduce_pro<1, 0, 2>(
+        n, pq[1], c_s, (const double*)nullptr, 0, BiGateStep{S, sid},
+        [=] __device__(const double* tt, double* sc, bool b0) {
+          const double sn = sqrt(tt[0]);
+          const bool conv = sn / S->bnorm < S->tol;
+          sc[0] = S->alpha;
+          sc[1] = conv ? 1.0 : 0.0;
+          if (b0) {
+            S->resnorm = sn;
+            if (S->collect) S->history[S->iters - 1] = sn;
+            if (conv) {
+              S->half_step = sid - 1;
+              S->converged = 1;
+              S->done = 1;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{t[i], sv[i], x[i], php[i]}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+          if (sc[1] != 0.0) {  // half-step exit (:271-278)
+            x[i] = (T)((double)e.c + sc[0] * (double)e.d);
+            return;
+          }
+          const double ti = (double)e.a;
+          acc[0] += ti * ti;
+          acc[1] += ti * (double)e.b;
+        }, pq[2]);
+    // K5 (:292-324)
+    c_r = launch_reduce_pro<2, 0, 2>(
+        n, pq[2], c_t, (const double*)nullptr, 0, BiGateNotDone{done},
+        [=] __device__(const double* tt, double* sc, bool b0) {
+          const double om = tt[0] == 0.0 ? 0.0 : tt[1] / tt[0];
+          sc[0] = S->alpha;
+          sc[1] = om;
+          if (b0) {
+            S->iters += 1;
+            S->omega = om;
+            S->rho = S->rho_new;
+            if (om == 0.0) {
+              S->breakdown = 1;
+              S->done = 1;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack8<T>{x[i], php[i], shp[i], sv[i], t[i],
+                                                    rh[i], T(0), T(0)}; },
+        [=] __device__(int64_t i, const Pack8<T>& e, double* acc, const double* sc) {
+          if (sc[1] == 0.0) return;
+          x[i] = (T)((double)e.a + (sc[0] * (double)e.b + sc[1] * (double)e.c));
+          const T ri = (T)((double)e.d - sc[1] * (double)e.e);
+          r[i] = ri;
+          acc[0] += (double)ri * (double)ri;
+          acc[1] += (double)e.f * (double)ri;
+        }, pq[3]);
+  };
+
   auto issue = [&](int64_t k) {
+    if (handoff) {
+      issue_handoff(k);
+      return;
+    }
     // rho_new = <r^, r>; beta (:203-206)
     launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
       acc[0] += (double)rh[i] * (double)r[i];
@@ -178,6 +364,17 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
   const int64_t steps = (cfg->max_iterations + 1) / 2;
   auto is_done = [&]() { return st.download().done != 0; };
   chunked_loop(steps, issue, is_done, 2, 32);
+  if (handoff && steps > 0) {
+    // the last step's |r| test (:310-324)
+    launch_reduce_pro<2, 0, 0>(
+        1, pq[3], c_r, (const double*)nullptr, 0, BiGateNotDone{done},
+        [=] __device__(const double* t, double*, bool b0) {
+          if (b0) bi_r_test(S, t[0]);
+        },
+        [=] __device__(int64_t) { return Pack4<T>{T(0), T(0), T(0), T(0)}; },
+        [=] __device__(int64_t, const Pack4<T>&, double*, const double*) {},
+        (double*)nullptr);
+  }
 
   h = st.download();
   out->residual_norm = h.resnorm;

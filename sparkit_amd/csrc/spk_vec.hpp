@@ -107,10 +107,14 @@ __device__ __forceinline__ void thread_sum_partials(const double* __restrict__ p
   for (int k = 0; k < W; ++k) v[k] = a[0][k] + tail[k];
 }
 
-// A four-wide element pack for prologue kernels' loads.
+// Four- and eight-wide element packs for prologue kernels' loads.
 template <typename T>
 struct Pack4 {
   T a, b, c, d;
+};
+template <typename T>
+struct Pack8 {
+  T a, b, c, d, e, f, g, h;
 };
 
 // Prologue: the totals of WI-wide partials (and of WJ-wide partials from a

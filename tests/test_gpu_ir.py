@@ -136,3 +136,33 @@ def test_cg_reliable_budget_and_exact_start(O, spk):
     with pytest.raises(spk.InvalidArgument):
         spk.ir_solve(spk.DeviceVector.from_numpy(bt), xd,
                      spk.IrConfig(inner_method=7), Ad)
+
+
+@pytest.mark.parametrize("prec", ["jacobi", "learned"])
+def test_cg_reliable_ill_conditioned(O, spk, prec):
+    """The FP32 recurrence with FP64 reliable updates on an ill-conditioned
+    anisotropic Q1 matrix (conductivity lognormal(0, 1.5), anisotropy up to
+    1e3; the small problems above are well conditioned): it reaches the
+    FP64 true-residual tolerance, with the learned inverse's symmetric part
+    in about the FP64 PCG's iteration count."""
+    A = spk.aniso_q1_2d(128, 4242, 1.5)
+    n = A.nrows
+    b = spk.uniform_pm1(42, n)
+    if prec == "jacobi":
+        M = spk.jacobi(A)
+    else:
+        mats = [spk.aniso_q1_2d(128, s, 1.5) for s in range(32)]
+        model = spk.SpaiModel(1)
+        model.train(mats, epochs=8, learning_rate=2e-3, shuffle_seed=7)
+        M = spk.spai(model.predict(A).symmetric_part())
+    x64 = spk.DeviceVector.zeros(n)
+    s64 = spk.conjugate_gradient(b, x64, spk.CGConfig(1e-10, 20000, 20000), A, M)
+    assert s64.converged
+    x = spk.DeviceVector.zeros(n)
+    cfg = spk.IrConfig(tolerance=1e-10, max_outer=100, inner_tolerance=1e-1,
+                       restart=0, max_inner=40000, inner_method=spk.IR_CG_RELIABLE)
+    s = spk.ir_solve(b, x, cfg, A, M)
+    assert s.converged and spk.true_relres(A, b, x) < 1e-10
+    if prec == "learned":
+        assert s.inner_iterations <= 1.2 * s64.computed_iterations + 10, (
+            s.inner_iterations, s64.computed_iterations)

@@ -365,3 +365,20 @@ def test_cg_with_explicit_approximate_inverse(O, spk, max_it):
         assert s.converged and sr.converged
         assert abs(s.computed_iterations - sr.computed_iterations) <= 1
         assert O.true_relres(A, b, x) < 1e-9
+
+
+@pytest.mark.parametrize("max_it", [2, 4, 6, 9, 14])
+@pytest.mark.parametrize("left,right", [(None, None), ("jacobi", None),
+                                        (None, "ilu")])
+def test_bicgstab_iterate_at_any_step(O, spk, max_it, left, right):
+    """BiCGSTAB stopped by its matvec budget at any step (each step spends
+    two; the half-step test on |s| and the step's |r| test both run) returns
+    the oracle's iterate and residual history (bicgstab.hpp:176-324)."""
+    A = O.convdiff_centered_2d(9, 8, 1.0, 0.4, 0.3, 2.0)
+    b = O.uniform_pm1(13, A.nrows)
+    x0 = O.uniform_pm1(14, A.nrows)
+    s, x, sr, xr = _run(spk, O, "bicgstab", A, b, 1e-30, left, right,
+                        max_it=max_it, x0=x0)
+    assert not s.converged and s.computed_iterations == sr.computed_iterations
+    assert np.allclose(x, xr, rtol=0, atol=1e-10 * np.abs(xr).max())
+    assert np.allclose(s.iteration_residuals, sr.iteration_residuals, rtol=1e-8)

@@ -236,3 +236,16 @@ def test_float_solvers(O):
     assert s32.converged and abs(s32.computed_iterations -
                                  s64.computed_iterations) <= 2
     assert np.allclose(x32, x64, rtol=0, atol=1e-4 * np.abs(x64).max())
+
+
+@pytest.mark.parametrize("budget", [1, 3, 9])
+def test_bicgstab_odd_budget_runs_a_whole_step(O, budget):
+    """BiCGSTAB spends two matvecs per step and tests the budget before a
+    step (bicgstab.hpp:176-178): an odd budget ends one matvec past it, with
+    one residual-history entry per matvec (|s| and |r| of every step)."""
+    A = O.convdiff_centered_2d(9, 8, 1.0, 0.4, 0.3, 2.0)
+    b = O.uniform_pm1(13, A.nrows)
+    x, s = O.bicgstab(A, b, tol=1e-30, max_iterations=budget)
+    assert s.computed_iterations == budget + 1
+    assert len(s.iteration_residuals) == budget + 1
+    assert np.all(np.isfinite(s.iteration_residuals))
