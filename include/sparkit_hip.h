@@ -536,7 +536,8 @@ spk_status spk_batch_create_diffusion(int64_t count, uint64_t seed,
                                       int64_t side_min, int64_t side_max,
                                       spk_batch* out);
 /* a batch over a copy of a block-diagonal store; row_offsets has count + 1
- * entries; SPK_INVALID_ARGUMENT if an entry couples two systems */
+ * entries; SPK_INVALID_ARGUMENT if an entry couples two systems or a
+ * system holds 2^29 rows or more */
 spk_status spk_batch_create(spk_csr A, int64_t count, const int64_t* row_offsets,
                             spk_batch* out);
 spk_status spk_batch_destroy(spk_batch B);

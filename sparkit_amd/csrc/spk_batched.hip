@@ -547,6 +547,12 @@ struct spk_batch_s {
   void setup() {
     hipStream_t s = stream();
     const size_t n = (size_t)total;
+    // the inner loop addresses a system's vectors with 32-bit byte offsets
+    int64_t max_rows = 0;
+    for (int64_t k = 0; k < K; ++k)
+      max_rows = std::max(max_rows, h_row[(size_t)k + 1] - h_row[(size_t)k]);
+    SPK_REQUIRE(max_rows < ((int64_t)1 << 29), SPK_INVALID_ARGUMENT,
+                "a system of the batch holds 2^29 rows or more");
     sys_row.alloc((size_t)K + 1);
     SPK_HIP_TRY(hipMemcpyAsync(sys_row.p, h_row.data(), (K + 1) * 8,
                                hipMemcpyHostToDevice, s));
@@ -592,8 +598,10 @@ struct spk_batch_s {
     int w = 0;
     SPK_HIP_TRY(hipMemcpyAsync(&w, mx.p, 4, hipMemcpyDeviceToHost, s));
     SPK_HIP_TRY(hipStreamSynchronize(s));
+    // ... and a system's sliced-ELL slots likewise (FP64 values: 8 bytes)
     if (w >= 1 && ell_width_for(w) > 0 &&
-        (int64_t)total * ell_width_for(w) < ((int64_t)1 << 31)) {
+        (int64_t)total * ell_width_for(w) < ((int64_t)1 << 31) &&
+        max_rows * ell_width_for(w) * 8 < ((int64_t)1 << 32)) {
       w = ell_width_for(w);
       ell_w = w;
       ell_ci.alloc(n * (size_t)w);
