@@ -21,6 +21,25 @@ struct BicgState {
   int64_t iters, steps;
   int32_t done, converged, breakdown, collect;
   double* history;
+  double rho_b[2];  // one-kernel hand-off: rho by step parity
+};
+
+struct BicgGateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+
+template <typename T>
+struct EpiBicgQParts {  // q = A p, partial <p~, q> (summed by the next kernel)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ pt;
+  T* __restrict__ q;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    q[i] = s;
+    return (double)pt[i] * (double)s;
+  }
+  __device__ void finalize(double) const {}
 };
 
 template <typename T>
@@ -122,9 +141,111 @@ void bicg_solve(spk_csr_s* A, spk_csr_s* At, spk_precond_s* ML, spk_precond_s* M
     p[i] = z[i];
     pt[i] = zt[i];
     acc[0] += (double)rt[i] * (double)z[i];
-  }, [=] __device__(const double* tt) { S->rho = tt[0]; });
+  }, [=] __device__(const double* tt) {
+    S->rho = tt[0];
+    S->rho_b[0] = tt[0];
+  });
 
-  auto issue = [&](int64_t) {
+  // One-kernel hand-off (the SpMV has few row blocks): no finisher
+  // kernels; every reduction's partials are summed by every block of the
+  // next kernel.  Per step: q = A p (partials <p~, q>); pass 1 (alpha;
+  // x, r; partials |r|^2); r~ -= alpha A^T p~; z, z~; pass 2 (the |r| test;
+  // partials rho_new = <r~, z>); pass 3 (beta; p, p~).  Block 0 records
+  // state no block of the same kernel reads (rho by step parity); a block
+  // 0 that ends the solve makes every block of its kernel skip its pass.
+  const bool handoff = spmv_partials_count(*A) <= kProMaxPartials &&
+                       pro_grid(n) <= kProMaxPartials;
+  double* pq[3] = {nullptr, nullptr, nullptr};
+  if (handoff) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    for (int i = 0; i < 3; ++i) pq[i] = (double*)ws.get(kWsBicgParts + i, pb);
+  }
+  auto issue_handoff = [&](int64_t k) {
+    const int par = (int)(k & 1);
+    const int c_q = launch_spmv_partials(*A, (const T*)p, EpiBicgQParts<T>{pt, q, done},
+                                         pq[0], s);
+    // alpha (:214-217); x += alpha p, r -= alpha q (:219-224)
+    const int c_r = launch_reduce_pro<1, 0, 1>(
+        n, pq[0], c_q, (const double*)nullptr, 0, BicgGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          sc[0] = S->rho_b[par] / t[0];
+          sc[1] = t[0] == 0.0 ? 1.0 : 0.0;
+          if (b0) {
+            if (t[0] == 0.0) {
+              S->breakdown = 1;
+              S->done = 1;
+            } else {
+              S->alpha = sc[0];
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{x[i], p[i], r[i], q[i]}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+          if (sc[1] != 0.0) return;
+          const double a = sc[0];
+          x[i] = (T)((double)e.a + a * (double)e.b);
+          const T ri = (T)((double)e.c - a * (double)e.d);
+          r[i] = ri;
+          acc[0] += (double)ri * (double)ri;
+        }, pq[1]);
+    launch_spmv(*At, (const T*)pt, EpiBicgShadow<T>{rt, S, done}, s);
+    precond(r, z, false);
+    precond(rt, zt, true);
+    // the |r| test (:226-238); rho_new = <r~, z> (:240-246)
+    const int c_rho = launch_reduce_pro<1, 0, 1>(
+        n, pq[1], c_r, (const double*)nullptr, 0, BicgGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double rn = sqrt(t[0]);
+          const bool conv = rn / S->bnorm < S->tol;
+          sc[0] = conv ? 1.0 : 0.0;
+          if (b0) {
+            S->iters += 2;
+            S->steps += 1;
+            S->resnorm = rn;
+            if (S->collect) S->history[S->steps - 1] = rn;
+            if (conv) {
+              S->converged = 1;
+              S->done = 1;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{rt[i], z[i], T(0), T(0)}; },
+        [=] __device__(int64_t, const Pack4<T>& e, double* acc, const double* sc) {
+          if (sc[0] != 0.0) return;
+          acc[0] += (double)e.a * (double)e.b;
+        }, pq[2]);
+    // beta (:246-250); p = z + beta p, p~ = z~ + beta p~ (:252-258)
+    launch_reduce_pro<1, 0, 0>(
+        n, pq[2], c_rho, (const double*)nullptr, 0, BicgGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double rn = t[0];
+          sc[0] = rn / S->rho_b[par];
+          sc[1] = rn == 0.0 ? 1.0 : 0.0;
+          if (b0) {
+            if (rn == 0.0) {
+              S->breakdown = 1;
+              S->done = 1;
+            } else {
+              S->beta = sc[0];
+              S->rho = rn;
+              S->rho_b[par ^ 1] = rn;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{z[i], p[i], zt[i], pt[i]}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+          if (sc[1] != 0.0) return;
+          const double be = sc[0];
+          p[i] = (T)((double)e.a + be * (double)e.b);
+          pt[i] = (T)((double)e.c + be * (double)e.d);
+        }, (double*)nullptr);
+  };
+
+  auto issue = [&](int64_t k) {
+    if (handoff) {
+      issue_handoff(k);
+      return;
+    }
     launch_spmv(*A, (const T*)p, EpiBicgQ<T>{pt, q, S, done}, s);
     launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
       const double a = S->alpha;

@@ -21,6 +21,25 @@ struct CgsState {
   int64_t iters, steps;
   int32_t done, converged, breakdown, collect;
   double* history;
+  double rho_b[2];  // one-kernel hand-off: rho by step parity
+};
+
+struct CgsGateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+
+template <typename T>
+struct EpiCgsQParts {  // q^ = A p^, partial <r^, q^> (summed by the next kernel)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ rh;
+  T* __restrict__ qh;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    qh[i] = s;
+    return (double)rh[i] * (double)s;
+  }
+  __device__ void finalize(double) const {}
 };
 
 template <typename T>
@@ -121,7 +140,10 @@ void cgs_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     p[i] = ri;
     u[i] = ri;
     acc[0] += (double)ri * (double)ri;
-  }, [=] __device__(const double* tt) { S->rho = tt[0]; });
+  }, [=] __device__(const double* tt) {
+    S->rho = tt[0];
+    S->rho_b[0] = tt[0];
+  });
 
   auto compose = [&](const T* in, T* outv) -> const T* {
     const bool l = !is_identity(ML), rr = !is_identity(MR);
@@ -132,7 +154,105 @@ void cgs_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     MR->apply((const T*)tmp, outv);
     return outv;
   };
-  auto issue = [&](int64_t) {
+  // One-kernel hand-off (the SpMV has few row blocks): no finisher
+  // kernels; every reduction's partials are summed by every block of the
+  // next kernel.  Per step: q^ = A p^ (partials <r^, q^>); pass 1 (alpha;
+  // q, u^); t = A p^' with x, r updates (partials |r|^2); pass 2 (the |r|
+  // test; partials rho_new = <r^, r>); pass 3 (beta; u, p).  Block 0 records
+  // state no block of the same kernel reads (rho by step parity); a block
+  // 0 that ends the solve makes every block of its kernel skip its pass.
+  const bool handoff = spmv_partials_count(*A) <= kProMaxPartials &&
+                       pro_grid(n) <= kProMaxPartials;
+  double* pq[3] = {nullptr, nullptr, nullptr};
+  if (handoff) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    for (int i = 0; i < 3; ++i) pq[i] = (double*)ws.get(kWsCgsParts + i, pb);
+  }
+  auto issue_handoff = [&](int64_t k) {
+    const int par = (int)(k & 1);
+    const T* php = compose(p, ph);
+    const int c_q = launch_spmv_partials(*A, php, EpiCgsQParts<T>{rh, qh, done}, pq[0], s);
+    // alpha (:224-226); q = u - alpha q^, u^ = u + q (:228-236)
+    launch_reduce_pro<1, 0, 0>(
+        n, pq[0], c_q, (const double*)nullptr, 0, CgsGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double alpha = S->rho_b[par] / t[0];
+          sc[0] = alpha;
+          sc[1] = t[0] == 0.0 ? 1.0 : 0.0;
+          if (b0) {
+            if (t[0] == 0.0) {
+              S->breakdown = 1;
+              S->done = 1;
+            } else {
+              S->alpha = alpha;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{u[i], qh[i], T(0), T(0)}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+          if (sc[1] != 0.0) return;
+          const double qi = (double)e.a - sc[0] * (double)e.b;
+          q[i] = (T)qi;
+          uh[i] = (T)((double)e.a + (double)(T)qi);
+        }, (double*)nullptr);
+    const T* uhp = compose(uh, ph);
+    const int c_r = launch_spmv_partials(*A, uhp, EpiCgsR<T>{uhp, x, r, S, done}, pq[1], s);
+    // the |r| test (:258-270); rho_new = <r^, r> (:196-206)
+    const int c_rho = launch_reduce_pro<1, 0, 1>(
+        n, pq[1], c_r, (const double*)nullptr, 0, CgsGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double rn = sqrt(t[0]);
+          const bool conv = rn / S->bnorm < S->tol;
+          sc[0] = conv ? 1.0 : 0.0;
+          if (b0) {
+            S->iters += 2;
+            S->steps += 1;
+            S->resnorm = rn;
+            if (S->collect) S->history[S->steps - 1] = rn;
+            if (conv) {
+              S->converged = 1;
+              S->done = 1;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{rh[i], r[i], T(0), T(0)}; },
+        [=] __device__(int64_t, const Pack4<T>& e, double* acc, const double* sc) {
+          if (sc[0] != 0.0) return;
+          acc[0] += (double)e.a * (double)e.b;
+        }, pq[2]);
+    // beta (:206-209); u = r + beta q, p = u + beta (q + beta p) (:211-222)
+    launch_reduce_pro<1, 0, 0>(
+        n, pq[2], c_rho, (const double*)nullptr, 0, CgsGateNotDone{done},
+        [=] __device__(const double* t, double* sc, bool b0) {
+          const double rn = t[0];
+          sc[0] = rn / S->rho_b[par];
+          sc[1] = rn == 0.0 ? 1.0 : 0.0;
+          if (b0) {
+            if (rn == 0.0) {
+              S->breakdown = 1;
+              S->done = 1;
+            } else {
+              S->beta = sc[0];
+              S->rho = rn;
+              S->rho_b[par ^ 1] = rn;
+            }
+          }
+        },
+        [=] __device__(int64_t i) { return Pack4<T>{r[i], q[i], p[i], T(0)}; },
+        [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+          if (sc[1] != 0.0) return;
+          const double be = sc[0];
+          const double qi = (double)e.b;
+          const T ui = (T)((double)e.a + be * qi);
+          u[i] = ui;
+          p[i] = (T)((double)ui + be * (qi + be * (double)e.c));
+        }, (double*)nullptr);
+  };
+  auto issue = [&](int64_t k) {
+    if (handoff) {
+      issue_handoff(k);
+      return;
+    }
     const T* php = compose(p, ph);
     launch_spmv(*A, php, EpiCgsQ<T>{rh, qh, S, done}, s);
     launch_map(n, [=] __device__(int64_t i) {

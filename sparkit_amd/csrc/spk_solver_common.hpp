@@ -19,6 +19,8 @@ enum : int {
   kWsBicg = 75,      // 75..84
   kWsMinres = 85,    // 85..94
   kWsQmr = 95,       // 95..109
+  kWsCgsParts = 110,   // 110..112: CGS hand-off partials
+  kWsBicgParts = 113,  // 113..115: BiCG hand-off partials
 };
 
 

@@ -68,3 +68,24 @@ def test_preconditioned_and_fp32(O, spk, left, right):
                  mk[right](Ad))
     assert s.converged
     assert O.true_relres(A, b, x32.numpy().astype(np.float64)) < 1e-4
+
+
+@pytest.mark.parametrize("budget", [2, 4, 6, 10])
+@pytest.mark.parametrize("left,right", [(None, None), ("jacobi", None),
+                                        (None, "ilu")])
+def test_iterate_at_any_step(O, spk, budget, left, right):
+    """BiCG stopped by its matvec budget at any step returns the oracle's
+    iterate and residual history (bicg.hpp:190-258)."""
+    A = O.convdiff_centered_2d(9, 8, 1.0, 0.4, 0.3, 2.0)
+    b = O.uniform_pm1(17, A.nrows)
+    x0 = O.uniform_pm1(18, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    mk = {"jacobi": spk.jacobi, "ilu": spk.ilu0, None: lambda _A: spk.identity}
+    x = dv(spk, x0)
+    s = spk.bicg(dv(spk, b), x, spk.BicgConfig(1e-30, budget, True), Ad,
+                 Ad.transpose(), mk[left](Ad), mk[right](Ad))
+    xr, sr = O.bicg(A, b, x0, tol=1e-30, max_iterations=budget, left=left,
+                    right=right)
+    assert not s.converged and s.computed_iterations == sr.computed_iterations
+    assert np.allclose(x.numpy(), xr, rtol=0, atol=1e-10 * np.abs(xr).max())
+    assert np.allclose(s.iteration_residuals, sr.iteration_residuals, rtol=1e-8)

@@ -66,3 +66,25 @@ def test_preconditioned(O, spk, left, right):
     assert s.converged and sr.converged
     assert abs(s.computed_iterations - sr.computed_iterations) <= 2
     assert O.true_relres(A, b, x.numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("budget", [2, 4, 6, 10])
+@pytest.mark.parametrize("left,right", [(None, None), ("jacobi", None),
+                                        (None, "ilu")])
+def test_iterate_at_any_step(O, spk, budget, left, right):
+    """CGS stopped by its matvec budget at any step returns the oracle's
+    iterate and residual history (cgs.hpp:176-270): every pass that one
+    step's scalars reach ran, and none of the next step's."""
+    A = O.convdiff_centered_2d(9, 8, 1.0, 0.4, 0.3, 2.0)
+    b = O.uniform_pm1(15, A.nrows)
+    x0 = O.uniform_pm1(16, A.nrows)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    mk = {"jacobi": spk.jacobi, "ilu": spk.ilu0, None: lambda _A: spk.identity}
+    x = dv(spk, x0)
+    s = spk.cgs(dv(spk, b), x, spk.CgsConfig(1e-30, budget, True), Ad,
+                mk[left](Ad), mk[right](Ad))
+    xr, sr = O.cgs(A, b, x0, tol=1e-30, max_iterations=budget, left=left,
+                   right=right)
+    assert not s.converged and s.computed_iterations == sr.computed_iterations
+    assert np.allclose(x.numpy(), xr, rtol=0, atol=1e-10 * np.abs(xr).max())
+    assert np.allclose(s.iteration_residuals, sr.iteration_residuals, rtol=1e-8)
