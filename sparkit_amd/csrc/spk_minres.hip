@@ -17,12 +17,75 @@
 namespace spk {
 namespace {
 
+// the rotation state one step hands to the next
+struct MinresGivens {
+  double beta, cs_prev, sn_prev, cs, sn, phi_bar;
+};
+
 struct MinresState {
   double bnorm, tol, resnorm, beta, alpha, cs_prev, sn_prev, cs, sn;
   double phi, phi_bar, delta, eps, rho, inv_bn;
   int64_t iters;
   int32_t done, converged, breakdown, collect;
   double* history;
+  // one-kernel hand-off: the rotation state by step parity, and the step
+  // whose w pass ran (gates the update pass)
+  MinresGivens g[2];
+  int64_t k2_step;
+};
+
+// minres_givens' arithmetic as a pure function of the previous step's
+// rotation state, alpha and <w, u> (minres.hpp:278-360): every block of the
+// update pass derives the same values
+struct MinresRot {
+  double delta, eps, rho, phi, inv_bn, est, beta_new;
+  bool breakdown;
+};
+__device__ inline MinresRot minres_rot(const MinresGivens& G, double alpha, double wu,
+                                       MinresGivens* next) {
+  MinresRot r{};
+  r.beta_new = sqrt(wu);
+  r.eps = G.sn_prev * G.beta;
+  const double delta_1 = G.cs_prev * G.beta;
+  r.delta = G.cs * delta_1 + G.sn * alpha;
+  const double rho_bar = -G.sn * delta_1 + G.cs * alpha;
+  r.rho = sqrt(rho_bar * rho_bar + r.beta_new * r.beta_new);
+  r.breakdown = r.rho == 0.0;
+  if (r.breakdown) return r;
+  const double cs_new = rho_bar / r.rho, sn_new = r.beta_new / r.rho;
+  r.phi = cs_new * G.phi_bar;
+  next->phi_bar = -sn_new * G.phi_bar;
+  r.est = fabs(next->phi_bar);
+  r.inv_bn = r.beta_new != 0.0 ? 1.0 / r.beta_new : 0.0;
+  next->cs_prev = G.cs;
+  next->sn_prev = G.sn;
+  next->cs = cs_new;
+  next->sn = sn_new;
+  next->beta = r.beta_new;
+  return r;
+}
+
+struct MinGateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+struct MinGateStep {
+  const MinresState* st;
+  int64_t step;
+  __device__ bool operator()() const { return st->k2_step == step; }
+};
+
+template <typename T>
+struct EpiMinresWParts {  // w = A z, partial <z, w> (summed by the next kernel)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ z;
+  T* __restrict__ w;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    w[i] = s;
+    return (double)z[i] * (double)s;
+  }
+  __device__ void finalize(double) const {}
 };
 
 template <typename T>
@@ -144,6 +207,7 @@ void minres_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     S->sn_prev = 0.0;
     S->cs = 1.0;
     S->sn = 0.0;
+    S->g[0] = MinresGivens{S->beta, 1.0, 0.0, 1.0, 0.0, S->beta};
     if (S->beta == 0.0) {
       S->converged = 1;
       S->done = 1;
@@ -159,7 +223,99 @@ void minres_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
       d[i] = T(0);
       dp[i] = T(0);
     });
+    // One-kernel hand-off (diagonal preconditioners, few SpMV row blocks):
+    // w = A z (partials <z, w>); the w pass sums them (alpha) and writes
+    // partials <w, u>; the update pass sums those, and every block derives
+    // the rotations from the previous step's state (block 0 records the new
+    // state in the other parity slot).
+    const bool handoff = diag && spmv_partials_count(*A) <= kProMaxPartials &&
+                         pro_grid(n) <= kProMaxPartials;
+    double* pq[2] = {nullptr, nullptr};
+    if (handoff) {
+      const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+      for (int i = 0; i < 2; ++i) pq[i] = (double*)ws.get(kWsMinresParts + i, pb);
+    }
+    auto issue_handoff = [&](int64_t k) {
+      const int par = (int)(k & 1);
+      const int64_t step = k + 1;
+      const int c1 = launch_spmv_partials(*A, (const T*)z,
+                                          EpiMinresWParts<T>{z, w, done}, pq[0], s);
+      T* u = vp;
+      const T* vc = v;
+      const int c2 = launch_reduce_pro<1, 0, 1>(
+          n, pq[0], c1, (const double*)nullptr, 0, MinGateNotDone{done},
+          [=] __device__(const double* t, double* sc, bool b0) {
+            sc[0] = t[0];
+            sc[1] = S->g[par].beta;
+            if (b0) {
+              S->alpha = t[0];
+              S->k2_step = step;
+            }
+          },
+          [=] __device__(int64_t i) {
+            return Pack8<T>{w[i], vc[i], u[i], dl ? dl[i] : T(1), dr ? dr[i] : T(1),
+                            T(0), T(0), T(0)};
+          },
+          [=] __device__(int64_t i, const Pack8<T>& e, double* acc, const double* sc) {
+            const T wi = e.a - ((T)sc[0] * e.b + (T)sc[1] * e.c);
+            w[i] = wi;
+            T ui = dl ? e.d * wi : wi;
+            ui = dr ? e.e * ui : ui;
+            u[i] = ui;
+            acc[0] += (double)wi * (double)ui;
+          }, pq[1]);
+      T* dc = d;
+      T* dn = dp;
+      T* vn = vp;
+      launch_reduce_pro<1, 0, 0>(
+          n, pq[1], c2, (const double*)nullptr, 0, MinGateStep{S, step},
+          [=] __device__(const double* t, double* sc, bool b0) {
+            MinresGivens next{};
+            const MinresRot r = minres_rot(S->g[par], S->alpha, t[0], &next);
+            sc[0] = r.delta;
+            sc[1] = r.eps;
+            sc[2] = r.rho;
+            sc[3] = r.phi;
+            sc[4] = r.inv_bn;
+            sc[5] = r.breakdown ? 0.0 : 1.0;
+            if (b0) {
+              if (r.breakdown) {
+                S->breakdown = 1;
+                S->done = 1;
+              } else {
+                S->g[par ^ 1] = next;
+                S->resnorm = r.est;
+                if (S->collect) S->history[S->iters] = r.est;
+                S->iters += 1;
+                if (r.est / S->bnorm < S->tol) {
+                  S->converged = 1;
+                  S->done = 1;
+                } else if (r.beta_new == 0.0) {
+                  S->done = 1;
+                }
+              }
+            }
+          },
+          [=] __device__(int64_t i) {
+            return Pack8<T>{z[i], dc[i], dn[i], vn[i], w[i], x[i], T(0), T(0)};
+          },
+          [=] __device__(int64_t i, const Pack8<T>& e, double*, const double* sc) {
+            if (sc[5] == 0.0) return;
+            const T dnew = (e.a - (T)sc[0] * e.b - (T)sc[1] * e.c) / (T)sc[2];
+            dn[i] = dnew;
+            x[i] = e.f + (T)sc[3] * dnew;
+            const T inv = (T)sc[4];
+            z[i] = e.d * inv;
+            vn[i] = e.e * inv;
+          }, (double*)nullptr);
+      std::swap(d, dp);
+      std::swap(v, vp);
+    };
     auto issue = [&](int64_t k) {
+      if (handoff) {
+        issue_handoff(k);
+        return;
+      }
       // lanczos_step (:227-237)
       launch_spmv(*A, (const T*)z, EpiMinresW<T>{z, w, S, done}, s);
       // precondition_lanczos (:239-250) + the rotations (:252-275)

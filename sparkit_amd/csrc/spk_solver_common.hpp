@@ -21,6 +21,7 @@ enum : int {
   kWsQmr = 95,       // 95..109
   kWsCgsParts = 110,   // 110..112: CGS hand-off partials
   kWsBicgParts = 113,  // 113..115: BiCG hand-off partials
+  kWsMinresParts = 116,  // 116..117: MINRES hand-off partials
 };
 
 
