@@ -27,6 +27,13 @@
 // last pass does x += alpha p; p = z + beta p.  (Forming r - alpha q inside
 // that SpMV's gather instead of K2 was measured slower: two loads per
 // gathered column cost more than the pass and launch they save.)
+// When the SpMV has at most kProMaxPartials row blocks (and M_R is the
+// identity, M_L Jacobi or an explicit inverse) the finishers go: K2 sums
+// K1's partials itself and derives alpha, the last pass sums K2's (and the
+// preconditioner SpMV's) and derives |r|, the convergence test and beta.
+// Every block sums the same partials in the same order, so all derive the
+// same scalars; block 0 records them.  rz alternates between two slots by
+// step parity, as the kernel that reads rz_k writes rz_{k+1}.
 #include "spk_solver_common.hpp"
 
 namespace spk {
