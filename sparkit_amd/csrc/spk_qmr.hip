@@ -21,12 +21,78 @@
 namespace spk {
 namespace {
 
+// the smoothing state one step hands to the next
+struct QmrSmooth {
+  double rho, xi, gamma, eta, theta, tau;
+};
+
 struct QmrState {
   double bnorm, tol, resnorm, rho, xi, gamma, eta, theta, eps, beta, tau;
   double delta, f1, f2, coeff, rho_new;
   int64_t iters, steps;
   int32_t done, converged, breakdown, collect;
   double* history;
+  // one-kernel hand-off: the smoothing state by step parity, and the step
+  // whose v~ pass ran (gates the update pass)
+  QmrSmooth q[2];
+  int64_t k4_step;
+};
+
+// qmr_smooth's arithmetic as a pure function of the previous step's state
+// (qmr.hpp:303-368): every block of the update pass derives the same values
+__device__ inline void qmr_next(const QmrSmooth& Q, double beta, double rho_new,
+                                double xi_new, QmrSmooth* N, double* coeff) {
+  const double theta_new = rho_new / (Q.gamma * fabs(beta));
+  const double gamma_new = 1.0 / sqrt(1.0 + theta_new * theta_new);
+  N->tau = Q.tau * theta_new * gamma_new;
+  N->eta = -Q.eta * (Q.rho / beta) * (gamma_new * gamma_new) / (Q.gamma * Q.gamma);
+  *coeff = (Q.theta * gamma_new) * (Q.theta * gamma_new);
+  N->theta = theta_new;
+  N->gamma = gamma_new;
+  N->rho = rho_new;
+  N->xi = xi_new;
+}
+
+struct QmrGateNotDone {
+  const int* done;
+  __device__ bool operator()() const { return !*done; }
+};
+struct QmrGateStep {
+  const QmrState* st;
+  int64_t step;
+  __device__ bool operator()() const { return st->k4_step == step; }
+};
+
+template <typename T>
+struct EpiQmrZParts {  // z = M_R(M_L(A p)), partial <q, z> (summed next)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ dl;
+  const T* __restrict__ dr;
+  const T* __restrict__ q;
+  T* __restrict__ z;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    T zi = dl ? dl[i] * s : s;
+    zi = dr ? dr[i] * zi : zi;
+    z[i] = zi;
+    return (double)q[i] * (double)zi;
+  }
+  __device__ void finalize(double) const {}
+};
+
+template <typename T>
+struct EpiQmrWtParts {  // w~ = A^T u - beta w, partial |w~|^2 (summed next)
+  static constexpr bool kReduce = true;
+  const T* __restrict__ w;
+  T* __restrict__ wt;
+  const QmrState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    const T v = s - (T)st->beta * w[i];
+    wt[i] = v;
+    return (double)v * (double)v;
+  }
+  __device__ void finalize(double) const {}
 };
 
 template <typename T>
@@ -182,6 +248,7 @@ void qmr_solve(spk_csr_s* A, spk_csr_s* At, spk_precond_s* ML, spk_precond_s* MR
     S->theta = 0.0;
     S->eps = 0.0;
     S->tau = S->rho;
+    S->q[0] = QmrSmooth{S->rho, S->xi, 1.0, -1.0, 0.0, S->rho};
     if (S->rho == 0.0) {  // x is exact
       S->converged = 1;
       S->done = 1;
@@ -191,13 +258,153 @@ void qmr_solve(spk_csr_s* A, spk_csr_s* At, spk_precond_s* ML, spk_precond_s* MR
     }
   });
   h = st.download();
+  // One-kernel hand-off (diagonal preconditioners, few SpMV row blocks):
+  // per step  K2 (delta from the partials of the previous update pass;
+  // f1, f2; p, q, u)  K3 z = M A p (partials <q, z>)  K4 (epsilon, beta;
+  // v~ = z - beta v, partials |v~|^2)  K5 w~ = A^T u - beta w (partials
+  // |w~|^2)  K6 (rho_new, xi_new and the smoothing, derived by every block
+  // from the previous step's state; d, x, v, w; partials <w, v> for the
+  // next delta).  The smoothing state alternates between two slots by
+  // step parity; K6 is gated on K4's step stamp, as its block 0 may end
+  // the solve.
+  const bool handoff = diag && spmv_partials_count(*A) <= kProMaxPartials &&
+                       spmv_partials_count(*At) <= kProMaxPartials &&
+                       pro_grid(n) <= kProMaxPartials;
+  double* pq[4] = {nullptr, nullptr, nullptr, nullptr};
+  int c_delta = 0;
+  if (handoff && !h.done) {
+    const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
+    for (int i = 0; i < 4; ++i) pq[i] = (double*)ws.get(kWsQmrParts + i, pb);
+  }
   if (!h.done) {
-    launch_map(n, [=] __device__(int64_t i) {
-      v[i] = vt[i] * (T)(1.0 / S->rho);
-      w[i] = wt[i] * (T)(1.0 / S->xi);
-      d[i] = T(0);
-    });
+    if (handoff) {
+      c_delta = launch_reduce_pro<1, 0, 1>(
+          n, pq[0], 0, (const double*)nullptr, 0, QmrGateNotDone{done},
+          [=] __device__(const double*, double*, bool) {},
+          [=] __device__(int64_t i) { return Pack4<T>{vt[i], wt[i], T(0), T(0)}; },
+          [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double*) {
+            const T vi = e.a * (T)(1.0 / S->rho);
+            const T wi = e.b * (T)(1.0 / S->xi);
+            v[i] = vi;
+            w[i] = wi;
+            d[i] = T(0);
+            acc[0] += (double)wi * (double)vi;
+          }, pq[0]);
+    } else {
+      launch_map(n, [=] __device__(int64_t i) {
+        v[i] = vt[i] * (T)(1.0 / S->rho);
+        w[i] = wt[i] * (T)(1.0 / S->xi);
+        d[i] = T(0);
+      });
+    }
+    auto issue_handoff = [&](int64_t k) {
+      const bool first = k == 0;
+      const int par = (int)(k & 1);
+      const int64_t step = k + 1;
+      // K2: delta (:237-242), f1, f2 (:244-249), p, q, u (:263-281)
+      launch_reduce_pro<1, 0, 0>(
+          n, pq[0], c_delta, (const double*)nullptr, 0, QmrGateNotDone{done},
+          [=] __device__(const double* t, double* sc, bool b0) {
+            const double dl0 = t[0];
+            sc[2] = dl0 == 0.0 ? 1.0 : 0.0;
+            sc[0] = first ? 0.0 : (S->q[par].xi * dl0) / S->eps;
+            sc[1] = first ? 0.0 : (S->q[par].rho * dl0) / S->eps;
+            if (b0) {
+              if (dl0 == 0.0) {
+                S->breakdown = 1;
+                S->done = 1;
+              } else {
+                S->delta = dl0;
+              }
+            }
+          },
+          [=] __device__(int64_t i) { return Pack4<T>{v[i], w[i], p[i], q[i]}; },
+          [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
+            if (sc[2] != 0.0) return;
+            T pi, qi;
+            if (first) {
+              pi = e.a;
+              qi = e.b;
+            } else {
+              pi = e.a - (T)sc[0] * e.c;
+              qi = e.b - (T)sc[1] * e.d;
+            }
+            p[i] = pi;
+            q[i] = qi;
+            T ui = dr ? dr[i] * qi : qi;
+            u[i] = dl ? dl[i] * ui : ui;
+          }, (double*)nullptr);
+      // K3: z = M A p, <q, z> (:283-297)
+      const int c_eps = launch_spmv_partials(*A, (const T*)p,
+                                             EpiQmrZParts<T>{dl, dr, q, z, done}, pq[1], s);
+      // K4: epsilon, beta (:298-300); v~ = z - beta v (:301-306)
+      const int c_rho = launch_reduce_pro<1, 0, 1>(
+          n, pq[1], c_eps, (const double*)nullptr, 0, QmrGateNotDone{done},
+          [=] __device__(const double* t, double* sc, bool b0) {
+            const double beta = t[0] / S->delta;
+            sc[0] = beta;
+            if (b0) {
+              S->eps = t[0];
+              S->beta = beta;
+              S->k4_step = step;
+            }
+          },
+          [=] __device__(int64_t i) { return Pack4<T>{z[i], v[i], T(0), T(0)}; },
+          [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
+            const T vi = e.a - (T)sc[0] * e.b;
+            vt[i] = vi;
+            acc[0] += (double)vi * (double)vi;
+          }, pq[2]);
+      // K5: w~ = A^T u - beta w (:307-320)
+      const int c_xi = launch_spmv_partials(*At, (const T*)u,
+                                            EpiQmrWtParts<T>{w, wt, S, done}, pq[3], s);
+      // K6: smoothing (:303-368), d, x (:330-348), v, w (:370-382), <w, v>
+      c_delta = launch_reduce_pro<1, 1, 1>(
+          n, pq[2], c_rho, pq[3], c_xi, QmrGateStep{S, step},
+          [=] __device__(const double* t, double* sc, bool b0) {
+            const double rho_new = sqrt(t[0]), xi_new = sqrt(t[1]);
+            QmrSmooth nx{};
+            double coeff = 0.0;
+            qmr_next(S->q[par], S->beta, rho_new, xi_new, &nx, &coeff);
+            sc[0] = nx.eta;
+            sc[1] = coeff;
+            sc[2] = rho_new;
+            sc[3] = xi_new;
+            if (b0) {
+              S->q[par ^ 1] = nx;
+              S->iters += 2;
+              S->steps += 1;
+              const double est = fabs(nx.tau);
+              S->resnorm = est;
+              if (S->collect) S->history[S->steps - 1] = est;
+              if (est / S->bnorm < S->tol) {
+                S->converged = 1;
+                S->done = 1;
+              } else if (rho_new == 0.0 || xi_new == 0.0) {
+                S->breakdown = 1;
+                S->done = 1;
+              }
+            }
+          },
+          [=] __device__(int64_t i) {
+            return Pack8<T>{p[i], d[i], x[i], vt[i], wt[i], T(0), T(0), T(0)};
+          },
+          [=] __device__(int64_t i, const Pack8<T>& e, double* acc, const double* sc) {
+            const T di = (T)sc[0] * e.a + (T)sc[1] * e.b;
+            d[i] = di;
+            x[i] = e.c + di;
+            const T vi = e.d / (T)sc[2];
+            const T wi = e.e / (T)sc[3];
+            v[i] = vi;
+            w[i] = wi;
+            acc[0] += (double)wi * (double)vi;
+          }, pq[0]);
+    };
     auto issue = [&](int64_t k) {
+      if (handoff) {
+        issue_handoff(k);
+        return;
+      }
       const bool first = k == 0;
       // qmr_step (:237-261): delta = <w, v>
       launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {

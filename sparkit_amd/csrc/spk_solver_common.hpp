@@ -22,6 +22,7 @@ enum : int {
   kWsCgsParts = 110,   // 110..112: CGS hand-off partials
   kWsBicgParts = 113,  // 113..115: BiCG hand-off partials
   kWsMinresParts = 116,  // 116..117: MINRES hand-off partials
+  kWsQmrParts = 118,     // 118..121: QMR hand-off partials
 };
 
 
