@@ -30,6 +30,25 @@ struct BicgGateNotDone {
 };
 
 template <typename T>
+struct EpiBicgShadowZ {  // q~ = A^T p~; r~ -= alpha q~; z~ = M_L(M_R(r~)) diagonal
+  static constexpr bool kReduce = false;
+  T* __restrict__ rt;
+  T* __restrict__ zt;
+  const T* __restrict__ dl;
+  const T* __restrict__ dr;
+  const BicgState* st;
+  const int* done;
+  __device__ double row(int i, T s) const {
+    const T ri = (T)((double)rt[i] - st->alpha * (double)s);
+    rt[i] = ri;
+    T t = dr ? dr[i] * ri : ri;
+    zt[i] = dl ? dl[i] * t : t;
+    return 0.0;
+  }
+  __device__ void finalize(double) const {}
+};
+
+template <typename T>
 struct EpiBicgQParts {  // q = A p, partial <p~, q> (summed by the next kernel)
   static constexpr bool kReduce = true;
   const T* __restrict__ pt;
@@ -160,6 +179,12 @@ void bicg_solve(spk_csr_s* A, spk_csr_s* At, spk_precond_s* ML, spk_precond_s* M
     const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
     for (int i = 0; i < 3; ++i) pq[i] = (double*)ws.get(kWsBicgParts + i, pb);
   }
+  // diagonal preconditioner slots: z = M_R(M_L(r)) and z~ = M_L(M_R(r~))
+  // are written by the pass that forms r and by the shadow SpMV's epilogue
+  // (the same products as the apply kernels; a copy for identity slots)
+  const bool dcomp = is_diagonal(ML) && is_diagonal(MR);
+  const T* dml = dcomp ? diag_of<T>(ML) : nullptr;
+  const T* dmr = dcomp ? diag_of<T>(MR) : nullptr;
   auto issue_handoff = [&](int64_t k) {
     const int par = (int)(k & 1);
     const int c_q = launch_spmv_partials(*A, (const T*)p, EpiBicgQParts<T>{pt, q, done},
@@ -186,11 +211,19 @@ void bicg_solve(spk_csr_s* A, spk_csr_s* At, spk_precond_s* ML, spk_precond_s* M
           x[i] = (T)((double)e.a + a * (double)e.b);
           const T ri = (T)((double)e.c - a * (double)e.d);
           r[i] = ri;
+          if (dcomp) {
+            T t = dml ? dml[i] * ri : ri;
+            z[i] = dmr ? dmr[i] * t : t;
+          }
           acc[0] += (double)ri * (double)ri;
         }, pq[1]);
-    launch_spmv(*At, (const T*)pt, EpiBicgShadow<T>{rt, S, done}, s);
-    precond(r, z, false);
-    precond(rt, zt, true);
+    if (dcomp) {
+      launch_spmv(*At, (const T*)pt, EpiBicgShadowZ<T>{rt, zt, dml, dmr, S, done}, s);
+    } else {
+      launch_spmv(*At, (const T*)pt, EpiBicgShadow<T>{rt, S, done}, s);
+      precond(r, z, false);
+      precond(rt, zt, true);
+    }
     // the |r| test (:226-238); rho_new = <r~, z> (:240-246)
     const int c_rho = launch_reduce_pro<1, 0, 1>(
         n, pq[1], c_r, (const double*)nullptr, 0, BicgGateNotDone{done},

@@ -169,6 +169,13 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
           acc[1] += ri * ri;
         }, pq[3]);
   }
+  // diagonal preconditioner slots (at least one Jacobi): p^ and s^ are
+  // written by the passes that form p and s (M_R(M_L(.)) elementwise, the
+  // same products as the separate apply kernels)
+  const bool dcomp = is_diagonal(ML) && is_diagonal(MR) &&
+                     !(is_identity(ML) && is_identity(MR));
+  const T* dml = dcomp ? diag_of<T>(ML) : nullptr;
+  const T* dmr = dcomp ? diag_of<T>(MR) : nullptr;
   auto issue_handoff = [&](int64_t k) {
     const int64_t sid = k + 1;
     // K1 (:203-206, :232-238; the previous step's :310-324)
@@ -199,10 +206,15 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         [=] __device__(int64_t i) { return Pack4<T>{r[i], p[i], v[i], T(0)}; },
         [=] __device__(int64_t i, const Pack4<T>& e, double*, const double* sc) {
           if (sc[2] != 0.0) return;
-          p[i] = (T)((double)e.a + sc[0] * ((double)e.b - sc[1] * (double)e.c));
+          const T pi = (T)((double)e.a + sc[0] * ((double)e.b - sc[1] * (double)e.c));
+          p[i] = pi;
+          if (dcomp) {
+            T t = dml ? dml[i] * pi : pi;
+            ph[i] = dmr ? dmr[i] * t : t;
+          }
         }, (double*)nullptr);
     // S1 (:240-250)
-    const T* php = compose(p, ph);
+    const T* php = dcomp ? (const T*)ph : compose(p, ph);
     c_v = launch_spmv_partials(*A, php, EpiBiVParts<T>{rh, v, done}, pq[0], s);
     // K2 (:211, :252-258)
     c_s = launch_reduce_pro<1, 0, 1>(
@@ -220,10 +232,14 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         [=] __device__(int64_t i, const Pack4<T>& e, double* acc, const double* sc) {
           const T si = (T)((double)e.a - sc[0] * (double)e.b);
           sv[i] = si;
+          if (dcomp) {
+            T t = dml ? dml[i] * si : si;
+            sh[i] = dmr ? dmr[i] * t : t;
+          }
           acc[0] += (double)si * (double)si;
         }, pq[1]);
     // S2 (:280-290)
-    const T* shp = compose(sv, sh);
+    const T* shp = dcomp ? (const T*)sh : compose(sv, sh);
     launch_spmv(*A, shp, EpiStore<T>{t, done}, s);
     // K4 (:259-278, :292-297)
     c_t = launch_reduce_pro<1, 0, 2>(

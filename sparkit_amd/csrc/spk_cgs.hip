@@ -168,9 +168,16 @@ void cgs_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
     for (int i = 0; i < 3; ++i) pq[i] = (double*)ws.get(kWsCgsParts + i, pb);
   }
+  // diagonal preconditioner slots (at least one Jacobi): u^' = M u^ and the
+  // next p^ = M p are written by the passes that form u^ and p
+  // (M_R(M_L(.)) elementwise, the same products as the apply kernels)
+  const bool dcomp = is_diagonal(ML) && is_diagonal(MR) &&
+                     !(is_identity(ML) && is_identity(MR));
+  const T* dml = dcomp ? diag_of<T>(ML) : nullptr;
+  const T* dmr = dcomp ? diag_of<T>(MR) : nullptr;
   auto issue_handoff = [&](int64_t k) {
     const int par = (int)(k & 1);
-    const T* php = compose(p, ph);
+    const T* php = (dcomp && k > 0) ? (const T*)ph : compose(p, ph);
     const int c_q = launch_spmv_partials(*A, php, EpiCgsQParts<T>{rh, qh, done}, pq[0], s);
     // alpha (:224-226); q = u - alpha q^, u^ = u + q (:228-236)
     launch_reduce_pro<1, 0, 0>(
@@ -193,9 +200,14 @@ void cgs_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
           if (sc[1] != 0.0) return;
           const double qi = (double)e.a - sc[0] * (double)e.b;
           q[i] = (T)qi;
-          uh[i] = (T)((double)e.a + (double)(T)qi);
+          const T ui = (T)((double)e.a + (double)(T)qi);
+          uh[i] = ui;
+          if (dcomp) {
+            T t = dml ? dml[i] * ui : ui;
+            ph[i] = dmr ? dmr[i] * t : t;
+          }
         }, (double*)nullptr);
-    const T* uhp = compose(uh, ph);
+    const T* uhp = dcomp ? (const T*)ph : compose(uh, ph);
     const int c_r = launch_spmv_partials(*A, uhp, EpiCgsR<T>{uhp, x, r, S, done}, pq[1], s);
     // the |r| test (:258-270); rho_new = <r^, r> (:196-206)
     const int c_rho = launch_reduce_pro<1, 0, 1>(
@@ -245,7 +257,12 @@ void cgs_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
           const double qi = (double)e.b;
           const T ui = (T)((double)e.a + be * qi);
           u[i] = ui;
-          p[i] = (T)((double)ui + be * (qi + be * (double)e.c));
+          const T pi = (T)((double)ui + be * (qi + be * (double)e.c));
+          p[i] = pi;
+          if (dcomp) {
+            T t = dml ? dml[i] * pi : pi;
+            ph[i] = dmr ? dmr[i] * t : t;
+          }
         }, (double*)nullptr);
   };
   auto issue = [&](int64_t k) {
