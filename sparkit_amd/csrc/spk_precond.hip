@@ -444,6 +444,23 @@ void spk_precond_s::build_strict() {
 }
 
 template <typename T>
+struct EpiJacLastForward {  // y = r - L_s y_old; z_0 = y / d (z may alias r)
+  static constexpr bool kReduce = false;
+  const T* r;
+  T* __restrict__ y;
+  const T* __restrict__ d;
+  T* z;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const {
+    const T yi = r[i] - s;
+    y[i] = yi;
+    z[i] = yi / d[i];
+    return 0.0;
+  }
+  __device__ void finalize(double) const {}
+};
+
+template <typename T>
 struct EpiJacBackward {  // z = (y - U_s z_old) / d
   static constexpr bool kReduce = false;
   const T* __restrict__ y;
@@ -475,6 +492,14 @@ void jac_buffers<float>(spk_precond_s& P, float** a, float** b) {
   *a = scratch_vec(P.jb32a, P.n);
   *b = scratch_vec(P.jb32b, P.n);
 }
+// a third buffer: z_0 for an odd sweep count >= 3, whose last forward pass
+// gathers from b
+template <typename T>
+static T* jac_buffer_c(spk_precond_s& P);
+template <>
+double* jac_buffer_c<double>(spk_precond_s& P) { return scratch_vec(P.jb64c, P.n); }
+template <>
+float* jac_buffer_c<float>(spk_precond_s& P) { return scratch_vec(P.jb32c, P.n); }
 
 template <typename T>
 static const T* diag_u(spk_precond_s& P);
@@ -501,16 +526,23 @@ void spk_precond_s::jacobi_apply(const T* in, T* out) {
   jac_buffers<T>(*this, &a, &b);
   const T* d = diag_u<T>(*this);
   // forward: y_0 = r, y_j = r - L_s y_{j-1}; y_k ends in a
+  // backward: z_0 = y / d, z_j = (y - U_s z_{j-1}) / d; z_k ends in out
+  // z_0 is written by the last forward pass: into out for an even k (in
+  // may alias out: that pass reads in[i] only for its own row, before
+  // writing z_0[i], and gathers from b), into b for k = 1 (its one pass
+  // gathers in), into a third buffer for an odd k >= 3 (its last pass
+  // gathers from b)
+  T* z0 = k % 2 == 0 ? out : (k == 1 ? b : jac_buffer_c<T>(*this));
   const T* y = in;
   for (int j = 0; j < k; ++j) {
     T* dst = (k - j) % 2 == 1 ? a : b;
-    launch_spmv(*Ls, y, EpiResidual<T>{in, dst}, s);
+    if (j == k - 1)
+      launch_spmv(*Ls, y, EpiJacLastForward<T>{in, dst, d, z0}, s);
+    else
+      launch_spmv(*Ls, y, EpiResidual<T>{in, dst}, s);
     y = dst;
   }
-  // backward: z_0 = y / d, z_j = (y - U_s z_{j-1}) / d; z_k ends in out
-  // (in may alias out: it is no longer read)
-  T* z0 = k % 2 == 0 ? out : b;
-  launch_map(n, [=] __device__(int64_t i) { z0[i] = y[i] / d[i]; });
+  if (k == 0) launch_map(n, [=] __device__(int64_t i) { z0[i] = y[i] / d[i]; });
   const T* z = z0;
   for (int j = 0; j < k; ++j) {
     T* dst = (k - j) % 2 == 1 ? out : b;

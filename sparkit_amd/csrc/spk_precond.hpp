@@ -51,8 +51,8 @@ struct spk_precond_s {
   int jac_sweeps = 0;
   spk_csr_s* Ls = nullptr;
   spk_csr_s* Us = nullptr;
-  spk::DevBuf<double> du64, jb64a, jb64b;
-  spk::DevBuf<float> du32, jb32a, jb32b;
+  spk::DevBuf<double> du64, jb64a, jb64b, jb64c;
+  spk::DevBuf<float> du32, jb32a, jb32b, jb32c;
   bool du_has32 = false;
   // SPAI
   spk_csr_s* M = nullptr;  // not owned

@@ -152,7 +152,7 @@ def test_ilu0_auto_schedule_picks_levels_for_wide_levels(O, spk):
     assert S.schedule == "sync-free"
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 7])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 7])
 def test_ilu0_jacobi_sweeps_match_oracle(O, spk, k):
     """Approximate apply (k Jacobi sweeps per triangle; no reference
     counterpart): on stencil rows the device sums each row in storage order
