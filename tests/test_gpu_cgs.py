@@ -88,3 +88,21 @@ def test_iterate_at_any_step(O, spk, budget, left, right):
     assert not s.converged and s.computed_iterations == sr.computed_iterations
     assert np.allclose(x.numpy(), xr, rtol=0, atol=1e-10 * np.abs(xr).max())
     assert np.allclose(s.iteration_residuals, sr.iteration_residuals, rtol=1e-8)
+
+
+def test_exact_start_is_a_breakdown(O, spk):
+    """x0 exact: rho = <r^, r> = 0 and the first alpha's denominator is
+    zero as well (the reference divides by it, cgs.hpp:224-226); the device
+    stops with a breakdown before any update, like the oracle."""
+    A = O.convdiff_centered_2d(6, 5, 1.0, 0.3, 0.2, 2.0)
+    xt = O.uniform_pm1(3, A.nrows)
+    b = O.spmv(A, xt)
+    xr, sr = O.cgs(A, b, xt, tol=1e-12, max_iterations=50)
+    assert sr.status != 0 and sr.computed_iterations == 0
+    for left in (None, "jacobi"):
+        Ad = spk.CsrMatrix.from_oracle(A)
+        M = spk.jacobi(Ad) if left else spk.identity
+        x = dv(spk, xt)
+        s = spk.cgs(dv(spk, b), x, spk.CgsConfig(1e-12, 50), Ad, M)
+        assert s.breakdown and not s.converged and s.computed_iterations == 0
+        assert np.array_equal(x.numpy(), xt)
