@@ -170,7 +170,8 @@ void build_panels(spk_csr_s& A, int P) {
                      (const int32_t*)A.rp.p, (const int32_t*)A.ci.p, P, width,
                      start.p);
   SPK_HIP_TRY(hipGetLastError());
-  std::vector<int32_t> hl((size_t)n), hrp((size_t)n + 1);
+  DevBuf<int32_t> frp((size_t)n + 1);  // an inner panel's full-row offsets
+  std::vector<int32_t> hl((size_t)n), hrp((size_t)n + 1), crp, crow;
   try {
     for (int p = 0; p < P; ++p) {
       hipLaunchKernelGGL(panel_len_kernel, dim3(grid_for(n)), dim3(256), 0, s, n,
@@ -183,19 +184,47 @@ void build_panels(spk_csr_s& A, int P) {
         acc += hl[(size_t)i];
         hrp[(size_t)i + 1] = (int32_t)acc;
       }
+      // the first and last panels keep every row; the inner ones only their
+      // non-empty rows (same ci / v, offsets of the kept rows, a row map)
+      const bool inner = p > 0 && p + 1 < P;
+      crp.clear();
+      crow.clear();
+      if (inner) {
+        crp.push_back(0);
+        for (int64_t i = 0; i < n; ++i)
+          if (hl[(size_t)i] > 0) {
+            crow.push_back((int32_t)i);
+            crp.push_back(hrp[(size_t)i + 1]);
+          }
+      }
+      const int64_t m = inner ? (int64_t)crow.size() : n;
       auto* Q = new spk_csr_s();
       A.panels.push_back(Q);
-      Q->allocate(n, A.ncols, acc);
-      SPK_HIP_TRY(hipMemcpyAsync(Q->rp.p, hrp.data(), (n + 1) * 4,
+      Q->allocate(m, A.ncols, acc);
+      int32_t* dst_rp = inner ? frp.p : Q->rp.p;
+      SPK_HIP_TRY(hipMemcpyAsync(dst_rp, hrp.data(), (n + 1) * 4,
                                  hipMemcpyHostToDevice, s));
       if (acc > 0)
         hipLaunchKernelGGL(panel_copy_kernel, dim3((unsigned)((n * 64 + 255) / 256)),
                            dim3(256), 0, s, n, (const int32_t*)start.p, p,
                            (const int32_t*)A.ci.p, (const double*)A.v64.p,
-                           (const int32_t*)Q->rp.p, Q->ci.p, Q->v64.p);
+                           (const int32_t*)dst_rp, Q->ci.p, Q->v64.p);
       SPK_HIP_TRY(hipGetLastError());
-      Q->h_rp = hrp;
-      upload_plan(*Q, build_spmv_plan(Q->h_rp.data(), n, &Q->nlong));
+      if (inner) {
+        SPK_HIP_TRY(hipMemcpyAsync(Q->rp.p, crp.data(), (m + 1) * 4,
+                                   hipMemcpyHostToDevice, s));
+        if (m > 0) {
+          Q->rows.alloc((size_t)m);
+          SPK_HIP_TRY(hipMemcpyAsync(Q->rows.p, crow.data(), m * 4,
+                                     hipMemcpyHostToDevice, s));
+        }
+        Q->h_rp = crp;
+      } else {
+        Q->h_rp = hrp;
+      }
+      // the host vectors are reused next pass: the copies must be done
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      upload_plan(*Q, build_spmv_plan(Q->h_rp.data(), m, &Q->nlong));
     }
   } catch (...) {
     A.drop_panels();

@@ -44,6 +44,7 @@ struct CsrView {
   const T* __restrict__ v;
   const int2* __restrict__ plan;  // {first row, first nonzero} per block
   int nblocks;
+  const int32_t* __restrict__ rows;  // matrix row of each stored row (null: same)
 };
 
 }  // namespace spk
@@ -61,8 +62,13 @@ struct spk_csr_s {
   // [p W, (p+1) W) as a store of its own; an SpMV runs the panels in column
   // order, each continuing every row's running sum from the previous panel
   // (pacc), so the gathers of one launch stay inside an L2-sized window of
-  // x.  Values are copied when the panels are made.
+  // x.  Values are copied when the panels are made.  The inner panels (all
+  // but the first and the last) keep only their non-empty rows: `rows` maps
+  // a stored row to its matrix row, and such a panel adds its sums to the
+  // running sums in place, so a row costs nothing in a panel it has no
+  // entries in.
   std::vector<spk_csr_s*> panels;
+  spk::DevBuf<int32_t> rows;  // compressed store: matrix row of each row
   spk::DevBuf<double> pacc64;
   spk::DevBuf<float> pacc32;
 
@@ -96,11 +102,11 @@ inline float* spk_csr_s::panel_acc<float>() {
 
 template <>
 inline spk::CsrView<double> spk_csr_s::view<double>() {
-  return {rp.p, ci.p, v64.p, plan.p, nblocks};
+  return {rp.p, ci.p, v64.p, plan.p, nblocks, rows.p};
 }
 template <>
 inline spk::CsrView<float> spk_csr_s::view<float>() {
-  return {rp.p, ci.p, vals32(), plan.p, nblocks};
+  return {rp.p, ci.p, vals32(), plan.p, nblocks, rows.p};
 }
 
 namespace spk {
@@ -144,6 +150,17 @@ struct EpiResidual {
   __device__ void finalize(double) const {}
 };
 
+// y += A x, in place: the running sum arrives as init == y, so y is not
+// restrict-qualified
+template <typename T>
+struct EpiAccum {
+  static constexpr bool kReduce = false;
+  T* y;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const { y[i] = s; return 0.0; }
+  __device__ void finalize(double) const {}
+};
+
 template <typename Epi>
 struct EpiFinalize {  // adapts Epi::finalize to the finisher's fin(totals)
   Epi epi;
@@ -164,11 +181,13 @@ struct Pair<float> {
 };
 typedef int IdxPair __attribute__((ext_vector_type(2)));
 
-// init (may be null): each row's sum starts from init[row] (column panels)
+// init (may be null): each row's sum starts from init[row] (column panels;
+// init may be the epilogue's own output).  Stored row r is matrix row
+// A.rows[r] when the store is compressed.
 template <typename T, typename Epi>
 __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials,
-    const T* __restrict__ init) {
+    const T* init) {
   __shared__ T prod[kNnzBlock + 2];
   __shared__ double red[kSpmvBlock / 64];
   if (epi.done && *epi.done) return;
@@ -183,7 +202,10 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
       s += __builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
     s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
-    if (tid == 0) acc = epi.row(r0, init ? init[r0] + s : s);
+    if (tid == 0) {
+      const int ri = A.rows ? A.rows[r0] : r0;
+      acc = epi.row(ri, init ? init[ri] + s : s);
+    }
   } else {
     // lanes per row: enough rows to occupy the block, about eight products
     // per lane; short rows (stencils) get one lane each
@@ -208,10 +230,11 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
       }
     }
     const int g = tid / L, lane = tid % L;
-    int s0 = 0, s1 = 0;
+    int s0 = 0, s1 = 0, ri = r0 + g;
     if (g < nrows) {
       s0 = A.rp[r0 + g] - base + off;
       s1 = A.rp[r0 + g + 1] - base + off;
+      if (A.rows) ri = A.rows[r0 + g];
     }
     T xv0[U], xv1[U];
 #pragma unroll
@@ -230,9 +253,9 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     if (L == 1) {
       // storage-order sum (sparse_blas.hpp:26-31)
       if (g < nrows) {
-        T s = init ? init[r0 + g] : T(0);
+        T s = init ? init[ri] : T(0);
         for (int j = s0; j < s1; ++j) s += prod[j];
-        acc = epi.row(r0 + g, s);
+        acc = epi.row(ri, s);
       }
     } else {
       T s = T(0);
@@ -240,7 +263,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
         for (int j = s0 + lane; j < s1; j += L) s += prod[j];
       for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
       if (g < nrows && lane == 0)
-        acc = epi.row(r0 + g, init ? init[r0 + g] + s : s);
+        acc = epi.row(ri, init ? init[ri] + s : s);
     }
   }
   if constexpr (Epi::kReduce) {
@@ -264,6 +287,28 @@ void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
   }
 }
 
+// All column panels but the last: the first stores every row's sum, the
+// compressed inner ones add theirs in place.  Returns the running sums.
+template <typename T>
+const T* launch_panels(spk_csr_s& A, const T* x, const int* done, hipStream_t s) {
+  T* acc = A.panel_acc<T>();
+  for (size_t p = 0; p + 1 < A.panels.size(); ++p) {
+    spk_csr_s& Q = *A.panels[p];
+    if (p == 0) {
+      EpiStore<T> part;
+      part.y = acc;
+      part.done = done;
+      launch_spmv_one(Q, x, part, (const T*)nullptr, s);
+    } else if (Q.nblocks > 0) {
+      EpiAccum<T> part;
+      part.y = acc;
+      part.done = done;
+      launch_spmv_one(Q, x, part, (const T*)acc, s);
+    }
+  }
+  return acc;
+}
+
 // The reduction's block partials (A.nblocks of them, or the last panel's)
 // go to `parts` and no finisher runs: the next kernel sums them itself.
 template <typename T, typename Epi>
@@ -272,16 +317,7 @@ int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts
   static_assert(Epi::kReduce, "an epilogue with a sum");
   spk_csr_s& last = A.panels.empty() ? A : *A.panels.back();
   const T* init = nullptr;
-  if (!A.panels.empty()) {
-    T* acc = A.panel_acc<T>();
-    for (size_t p = 0; p + 1 < A.panels.size(); ++p) {
-      EpiStore<T> part;
-      part.y = acc;
-      part.done = epi.done;
-      launch_spmv_one(*A.panels[p], x, part, init, s);
-      init = acc;
-    }
-  }
+  if (!A.panels.empty()) init = launch_panels(A, x, epi.done, s);
   hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(last.nblocks), dim3(kSpmvBlock),
                      0, s, last.view<T>(), x, epi, parts, init);
   SPK_HIP_TRY(hipGetLastError());
@@ -300,15 +336,7 @@ void launch_spmv(spk_csr_s& A, const T* x, const Epi& epi, hipStream_t s) {
     launch_spmv_one(A, x, epi, (const T*)nullptr, s);
     return;
   }
-  T* acc = A.panel_acc<T>();
-  const T* init = nullptr;
-  for (size_t p = 0; p + 1 < A.panels.size(); ++p) {
-    EpiStore<T> part;
-    part.y = acc;
-    part.done = epi.done;
-    launch_spmv_one(*A.panels[p], x, part, init, s);
-    init = acc;
-  }
+  const T* init = launch_panels(A, x, epi.done, s);
   launch_spmv_one(*A.panels.back(), x, epi, init, s);
 }
 
