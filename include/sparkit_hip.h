@@ -195,9 +195,9 @@ spk_status spk_csr_permute(spk_csr A, const int64_t* d_perm, int mode,
  * them.  No reference counterpart (Compressed_row_matrix has one layout). */
 spk_status spk_csr_set_panels(spk_csr A, int64_t panels);
 spk_status spk_csr_get_panels(spk_csr A, int64_t* panels);
-/* times one FP64 SpMV for 1, 4, 8, ... 64 panels (those leaving at least
- * 32768 columns per panel), keeps the fastest; tried / tried_us (may be NULL,
- * room for 6) list the candidates */
+/* times one FP64 SpMV for 1, 4, 8, 12, 16, 24, 32 and 64 panels (those
+ * leaving at least 32768 columns per panel), keeps the fastest; tried /
+ * tried_us (may be NULL, room for 8) list the candidates */
 spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
                                float* tried_us, int64_t* ntried);
 

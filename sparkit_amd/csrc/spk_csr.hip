@@ -432,7 +432,7 @@ spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
     DevBuf<double> y((size_t)std::max<int64_t>(A->nrows, 1));
     SPK_HIP_TRY(hipMemsetAsync(x.p, 0, x.n * 8, s));
     std::vector<int> cands{1};
-    for (int P : {4, 8, 16, 32, 64})
+    for (int P : {4, 8, 12, 16, 24, 32, 64})
       if (A->ncols >= (int64_t)P * 32768) cands.push_back(P);
     hipEvent_t e0, e1;
     SPK_HIP_TRY(hipEventCreate(&e0));
