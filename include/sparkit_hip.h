@@ -190,7 +190,8 @@ spk_status spk_csr_permute(spk_csr A, const int64_t* d_perm, int mode,
  * share of the columns; an SpMV runs the panels in column order, carrying
  * every row's running sum from one to the next, so each launch gathers from
  * an L2-sized window of x (for matrices whose rows scatter over all columns;
- * rows summed by one lane keep the reference's storage-order sum).  The
+ * rows summed by one lane keep the reference's storage-order sum).  Panels
+ * other than the first and the last keep only their non-empty rows.  The
  * values are copied when the panels are made: call again after changing
  * them.  No reference counterpart (Compressed_row_matrix has one layout). */
 spk_status spk_csr_set_panels(spk_csr A, int64_t panels);

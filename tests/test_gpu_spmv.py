@@ -259,3 +259,32 @@ def test_spmv_odd_block_starts(O, spk, case):
     y32 = spk.DeviceVector.zeros(A.nrows, np.float32)
     Ad.spmv(spk.DeviceVector.from_numpy(x32), y32)
     assert np.array_equal(y32.numpy(), O.spmv(A, x32))
+
+
+def test_panels_with_few_and_no_rows(O, spk):
+    """Inner column panels keep only their non-empty rows: here panel 1
+    holds entries of every 37th row only and panel 2 none at all.  The
+    running sums still follow column order, so one-lane rows stay
+    bit-identical to the oracle, in FP64 and FP32."""
+    n = 4000
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        rows += [i, i]
+        cols += [i % 1000, 3000 + (7 * i) % 1000]
+        vals += [1.0 + i % 5, -0.5 - i % 3]
+        if i % 37 == 0:
+            rows.append(i)
+            cols.append(1000 + (3 * i) % 1000)
+            vals.append(0.25 + i % 7)
+    A = O.make_matrix(n, n, rows, cols, vals)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    Ad.set_panels(4)
+    assert Ad.panels == 4
+    x = O.uniform_pm1(23, n)
+    y = spk.DeviceVector.zeros(n)
+    Ad.spmv(spk.DeviceVector.from_numpy(x), y)
+    assert np.array_equal(y.numpy(), O.spmv(A, x))
+    x32 = x.astype(np.float32)
+    y32 = spk.DeviceVector.zeros(n, np.float32)
+    Ad.spmv(spk.DeviceVector.from_numpy(x32), y32)
+    assert np.array_equal(y32.numpy(), O.spmv(A, x32))
