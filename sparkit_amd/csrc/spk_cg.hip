@@ -76,6 +76,21 @@ struct EpiCgDot {  // q = A p^, partial <p, q>
 };
 
 template <typename T>
+struct EpiCgDotRzb {  // q = A p^, alpha = <r,z> / <p,q> from the parity buffer
+  static constexpr bool kReduce = true;
+  const T* __restrict__ p;
+  T* __restrict__ q;
+  CgState* st;
+  const int* done;
+  int par;
+  __device__ double row(int i, T s) const {
+    q[i] = s;
+    return (double)p[i] * (double)s;
+  }
+  __device__ void finalize(double pq) const { st->alpha = st->rzb[par] / pq; }
+};
+
+template <typename T>
 struct EpiCgRestartJacobi {  // r = b - A x; z = D^-1 r; p = z; <r, z>
   static constexpr bool kReduce = true;
   const T* __restrict__ b;
@@ -194,10 +209,14 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
                        spmv_partials_count(*A) <= kProMaxPartials &&
                        (!spmv_left || spmv_partials_count(*ML->M) <= kProMaxPartials) &&
                        pro_grid(n) <= kProMaxPartials;
+  // Jacobi-PCG whose SpMV has too many row blocks for a prologue: the SpMV
+  // keeps its finisher (alpha), the r update hands its partials to the
+  // x / p pass
+  const bool mixed = !handoff && right_id && jac && pro_grid(n) <= kProMaxPartials;
   double* part1 = nullptr;
   double* part2 = nullptr;
   double* part3 = nullptr;
-  if (handoff) {
+  if (handoff || mixed) {
     const size_t pb = sizeof(double) * 2 * (size_t)kProMaxPartials;
     part1 = (double*)ws.get(kWsCg + 7, pb);
     part2 = (double*)ws.get(kWsCg + 8, pb);
@@ -223,20 +242,25 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         }, done);
       }
     }
-    if (handoff) {
+    if (handoff || mixed) {
       // One-kernel hand-off: no finisher kernels.  K1 q = A p (partials
       // <p,q>); K2 sums them itself -> alpha, r -= alpha q, partials;
       // [K3 z = M r, partials <r,z>]; last pass sums the partials ->
-      // |r|, convergence, beta; x += alpha p; p = z + beta p.
+      // |r|, convergence, beta; x += alpha p; p = z + beta p.  Mixed: K1
+      // ends with a finisher that stores alpha, which K2 reads.
       const int par = (int)(k & 1);
       const int64_t step = k + 1;
-      const int c1 = launch_spmv_partials(*A, (const T*)p, EpiCgDot<T>{p, q, S, done},
-                                          part1, s);
+      int c1 = 0;
+      if (handoff)
+        c1 = launch_spmv_partials(*A, (const T*)p, EpiCgDot<T>{p, q, S, done}, part1, s);
+      else
+        launch_spmv(*A, (const T*)p, EpiCgDotRzb<T>{p, q, S, done, par}, s);
+      const bool ho = handoff;
       auto pro_alpha = [=] __device__(const double* t, double* sc, bool b0) {
-        const double alpha = S->rzb[par] / t[0];
+        const double alpha = ho ? S->rzb[par] / t[0] : S->alpha;
         sc[0] = alpha;
         if (b0) {
-          S->alpha = alpha;
+          if (ho) S->alpha = alpha;
           S->k2_step = step;
         }
       };
