@@ -34,6 +34,27 @@ def test_poisson_192_spmv_bit_exact_and_pcg(O, spk):
     assert spk.true_relres(A, b, xs) < 1e-8
 
 
+@pytest.mark.parametrize("max_it", [5, 6])
+def test_poisson_192_pcg_iterate_at_step(O, spk, max_it):
+    """Above 4096 SpMV row blocks Jacobi-PCG keeps the SpMV's finisher and
+    hands the r update's partials to the x / p pass: a solve stopped at a
+    restart step and one past it return the oracle's iterate and residual
+    history (conjugate_gradient.hpp:208-292)."""
+    print(f"full size: poisson_3d(192) Jacobi-PCG stopped after {max_it}", flush=True)
+    R = O.poisson_3d(192)
+    A = spk.CsrMatrix.from_oracle(R)
+    b = O.uniform_pm1(11, R.nrows)
+    x0 = O.uniform_pm1(12, R.nrows)
+    xd = _dv(spk, x0)
+    cfg = spk.CGConfig(1e-30, 5, max_it, True, spk.F64)
+    s = spk.conjugate_gradient(_dv(spk, b), xd, cfg, A, spk.jacobi(A), spk.identity)
+    xr, sr = O.cg(R, b, x0, tol=1e-30, restart_iterations=5,
+                  max_iterations=max_it, left="jacobi", right=None)
+    assert not s.converged and s.computed_iterations == max_it
+    assert np.allclose(xd.numpy(), xr, rtol=0, atol=1e-11 * np.abs(xr).max())
+    assert np.allclose(s.iteration_residuals, sr.iteration_residuals, rtol=1e-9)
+
+
 def test_powerlaw_4m_row_sums_symmetry_and_pcg(spk):
     """Off-diagonals in [-1, 0] and diagonal = sum |off| + 1 (matgen.hpp:
     657-658): every row of A sums to 1 up to rounding; A is symmetric, so
