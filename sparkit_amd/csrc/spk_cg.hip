@@ -33,7 +33,9 @@
 // preconditioner SpMV's) and derives |r|, the convergence test and beta.
 // Every block sums the same partials in the same order, so all derive the
 // same scalars; block 0 records them.  rz alternates between two slots by
-// step parity, as the kernel that reads rz_k writes rz_{k+1}.
+// step parity, as the kernel that reads rz_k writes rz_{k+1}.  Jacobi-PCG
+// with more SpMV row blocks than that keeps K1's finisher (alpha from the
+// parity slot) and hands off from K2 on.
 #include "spk_solver_common.hpp"
 
 namespace spk {
