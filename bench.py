@@ -31,6 +31,7 @@ cpu_baseline: the oracle's Jacobi-PCG (CPU restatement of the reference) on
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -40,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the network's output layer in the canonical parameter order
+# W1[24][64] b1[64] W2[64][64] b2[64] W3[64][9] b3[9]
+SPAI_W3 = 24 * 64 + 64 + 64 * 64 + 64
+SPAI_B3 = SPAI_W3 + 64 * 9
 # HBM-side bytes per SpMV measured with rocprofv3 PMC counters (FETCH_SIZE x
 # 2.0 + WRITE_SIZE, calibrated on a 256 MiB copy) for the default
 # configurations; the commands and the raw counts are in
@@ -62,24 +67,126 @@ def b_pcg(n, nnz, s=8):
     return b_spmv(n, nnz, s) + 11 * s * n
 
 
-def cpu_baseline(grid, iters):
-    """Oracle Jacobi-PCG, `iters` iterations on input (a) (1 core)."""
-    from oracle import oracle as O
-    A = O.poisson_3d(grid)
-    b = O.uniform_pm1(42, A.nrows)
-    t0 = time.perf_counter()
-    _, s = O.cg(A, b, tol=1e-30, restart_iterations=1 << 40,
-                max_iterations=iters, left="jacobi")
-    dt = time.perf_counter() - t0
-    n, nnz = A.nrows, A.nnz
-    bytes_ = s.computed_iterations * b_pcg(n, nnz)
-    return {"value": round(bytes_ / dt / 1e9, 3), "unit": "GB/s",
-            "cores": 1, "kind": "port",
-            "sample": f"{s.computed_iterations} Jacobi-PCG iterations of the "
-                      f"oracle on the {grid}^3 Poisson matrix ({dt:.2f} s, "
-                      f"{dt / s.computed_iterations * 1e3:.1f} ms/iteration; "
-                      f"a full 644-iteration solve would take "
-                      f"{dt / s.computed_iterations * 644:.0f} s)"}
+CPU_LEGS = ("poisson192", "powerlaw", "aniso", "convdiff", "batched", "p2d")
+
+
+def start_cpu_legs(legs):
+    """Starts the oracle's CPU-baseline legs (oracle/cpu_baseline.py) as
+    child processes before this process touches the GPU.  Each child pins
+    itself to one host core (as taskset -c does) and runs its legs one
+    after another; this process keeps the first two cores.  With fewer
+    than three cores every leg runs in one child on the last core."""
+    cores = sorted(os.sched_getaffinity(0))
+    spare = cores[2:] or cores[-1:]
+    groups = [legs[i::len(spare)] for i in range(min(len(spare), len(legs)))]
+    procs = []
+    for core, group in zip(spare, groups):
+        cmd = [sys.executable, "-m", "oracle.cpu_baseline", ",".join(group),
+               "--core", str(core)]
+        procs.append((group, subprocess.Popen(
+            cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+            text=True)))
+    if len(cores) > 2:
+        os.sched_setaffinity(0, set(cores[:2]))
+    return {"procs": procs, "t0": time.perf_counter(), "nproc": len(cores),
+            "cores": cores}
+
+
+def collect_cpu_legs(state, deadline_s):
+    """Waits for the legs (until deadline_s after they started); a leg
+    still running then is stopped and reported unfinished."""
+    out = {}
+    for group, p in state["procs"]:
+        left = max(1.0, deadline_s - (time.perf_counter() - state["t0"]))
+        try:
+            so, se = p.communicate(timeout=left)
+        except subprocess.TimeoutExpired:
+            p.kill()
+            so, se = p.communicate()
+        for line in so.splitlines():
+            try:
+                r = json.loads(line)
+                out[r["leg"]] = r
+            except (ValueError, KeyError):
+                pass
+        for leg in group:
+            if leg not in out:
+                out[leg] = {"leg": leg, "unfinished": True,
+                            "rc": p.returncode, "stderr": se[-300:]}
+    return out
+
+
+def attach_cpu_legs(out, legs):
+    """GPU time beside the CPU leg of the same configuration."""
+    def put(sec, leg, gpu_ms):
+        r = legs.get(leg, {})
+        if sec in out and isinstance(out[sec], dict) and r.get("s") and gpu_ms:
+            out[sec]["cpu_" + leg] = {"s": r["s"],
+                                      "extrapolated": bool(r.get("extrapolated")),
+                                      "gpu_speedup": round(r["s"] * 1e3 / gpu_ms, 1)}
+    p2 = out.get("poisson2d_pcg", {})
+    put("poisson2d_pcg", "p2d", p2.get("ms"))
+    mp = out.get("mixed_precision", {})
+    put("mixed_precision", "convdiff",
+        mp.get("fp64_gmres50_ilu0_mgs", {}).get("ms"))
+    ls = out.get("learned_spai", {})
+    put("learned_spai", "aniso", ls.get("jacobi_pcg_fp64", {}).get("ms"))
+    bp = out.get("batched_pcg", {})
+    put("batched_pcg", "batched", bp.get("fp64_graph", {}).get("ms"))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(legs, state, inputs):
+    """The headline's CPU figure: the oracle's Jacobi-PCG on input (a)
+    (full solve) and input (b) (extrapolated from a sample), in the
+    metric's unit, beside every leg's own report."""
+    a, b = legs.get("poisson192", {}), legs.get("powerlaw", {})
+    nproc = state["nproc"]
+    res = {"unit": "GB/s", "cores": 1, "kind": "port",
+           "cpu_model": cpu_model(), "nproc": nproc,
+           "cores_note": f"1 of {nproc} cores per leg; legs ran at the same "
+                         f"time, each pinned to its own core",
+           "build": "oracle/liboracle.so, gcc -O3 -DNDEBUG -ffp-contract=off, "
+                    "operator slots allocating and copying per call as the "
+                    "reference's multiply / ilu_apply do",
+           "legs": legs}
+    if a.get("s") and b.get("s"):
+        byt = 0.0
+        for inp in inputs:
+            leg = a if inp.name.startswith("3D") else b
+            it = leg.get("iterations", POWERLAW_ITERS_CPU)
+            byt += it * b_pcg(inp.n, inp.nnz)
+        res["value"] = round(byt / (a["s"] + b["s"]) / 1e9, 3)
+        res["sample"] = (f"Jacobi-PCG to 1e-8: (a) full solve, "
+                         f"{a.get('iterations')} iterations in {a['s']:.1f} s; "
+                         f"(b) extrapolated: {b.get('solver')} on a "
+                         f"{b.get('sample_rows')}-row sample of the same law "
+                         f"({b.get('sample_s')} s) scaled {b.get('scaling')} "
+                         f"-> {b['s']:.1f} s")
+        res["solve_s"] = round(a["s"] + b["s"], 3)
+    elif a.get("s"):
+        inp = [i for i in inputs if i.name.startswith("3D")][0]
+        res["value"] = round(a["iterations"] * b_pcg(inp.n, inp.nnz) /
+                             a["s"] / 1e9, 3)
+        res["sample"] = (f"input (a) only: Jacobi-PCG full solve, "
+                         f"{a['iterations']} iterations in {a['s']:.1f} s")
+    else:
+        res["value"] = None
+        res["sample"] = "the CPU legs did not finish"
+    return res
+
+
+POWERLAW_ITERS_CPU = 11  # the power-law solve's Jacobi-PCG count
 
 
 class Input:
@@ -124,6 +231,21 @@ def _mix(seed, k):
     z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
     z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
     return z ^ (z >> 31)
+
+
+def speedup(res, base, other, target=1e-8):
+    """base time / other time when both runs converged to the same FP64
+    true-residual target; otherwise None, with the failed runs named."""
+    a, b = res.get(base, {}), res.get(other, {})
+    ok = [r.get("converged") and r.get("true_relres", 1.0) < target * 1.01
+          for r in (a, b)]
+    if all(ok):
+        return round(a["ms"] / b["ms"], 3)
+    failed = {k: {"converged": r.get("converged"),
+                  "true_relres": r.get("true_relres")}
+              for k, r, o in ((base, a, ok[0]), (other, b, ok[1])) if not o}
+    res.setdefault("speedup_not_comparable", {})[f"{other}_vs_{base}"] = failed
+    return None
 
 
 def _timed(fn):
@@ -217,7 +339,9 @@ def mixed_precision_section(spk, grid=128, seed=42):
             "converged": sir.converged, "ms": round(tir, 2), "true_relres": rir,
             "rel_error_vs_fp64_solve": float(np.linalg.norm(air - a64) /
                                              np.linalg.norm(a64))}
-        res[f"speedup_ir_vs_fp64_{tag}_{oname}"] = round(t64 / tir, 3)
+        res[f"speedup_ir_vs_fp64_{tag}_{oname}"] = speedup(
+            res, f"fp64_gmres50_{tag}_{oname}", f"ir_fp32_gmres50_{tag}_{oname}",
+            1e-10)
         log(f"mixed precision ({tag}, {oname}): FP64 GMRES "
             f"{s64.computed_iterations} it {t64:.1f} ms relres {r64:.2e}; IR "
             f"{sir.outer_iterations} outer {sir.inner_iterations} inner "
@@ -241,8 +365,10 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     anisotropy log-uniform [1,1e3], random rotation), trained on the device
     on train_count matrices of train_side^2 (seeds 0..train_count-1);
     compared with Jacobi-PCG, ILU(0)-PCG and FGMRES(30) with the untrained
-    (Jacobi) network, each to a 1e-8 FP64 true residual.  FGMRES orthogonalises by
-    CGS2 (one MGS run is reported beside it)."""
+    (Jacobi) network, each to a 1e-8 FP64 true residual; FGMRES
+    orthogonalises by CGS2.  The guarded solve (spk_learned_solve) runs the
+    same FGMRES behind the check of M and the stall monitor, and once with a
+    poisoned network, whose M it rejects before falling back to Jacobi-PCG."""
     t0 = time.perf_counter()
     mats = [spk.aniso_q1_2d(train_side, s, 1.5) for s in range(train_count)]
     t_gen = time.perf_counter() - t0
@@ -289,7 +415,6 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
 
     cg = spk.CGConfig(1e-8, 50000, 50000)
     fg = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_CGS2)
-    fgm = spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64, spk.ORTHO_MGS)
     run("jacobi_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cg, A, J))
     # the learned inverse's symmetric part (SPD here) in the CG slot, FP64
     # and as FP32 PCG with FP64 reliable updates
@@ -307,10 +432,9 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                          inner_method=spk.IR_CG_RELIABLE)
     run("learned_sym_relcg_fp32_update_every_10x",
         lambda x: spk.ir_solve(b, x, rcfg1, A, Ms))
-    jt = res["jacobi_pcg_fp64"]["ms"]
     for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32",
               "learned_sym_relcg_fp32_update_every_10x"):
-        res[f"speedup_{k}_vs_jacobi_pcg"] = round(jt / res[k]["ms"], 3)
+        res[f"speedup_{k}_vs_jacobi_pcg"] = speedup(res, "jacobi_pcg_fp64", k)
     if pcg_only:  # the PCG comparisons only (no ILU(0)-PCG, no FGMRES)
         return res
     F, t_ilu = _timed(lambda: spk.ilu0(A))
@@ -322,18 +446,41 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     run("ilu0_pcg_fp64", lambda x: spk.conjugate_gradient(b, x, cgi, A, F),
         warm=False)
     run("fgmres30_learned_fp64", lambda x: spk.fgmres(b, x, fg, A, Ml))
-    run("fgmres30_learned_fp64_mgs", lambda x: spk.fgmres(b, x, fgm, A, Ml))
-    run("fgmres30_untrained_jacobi_fp64", lambda x: spk.fgmres(b, x, fg, A, Mj))
-    icfg = spk.IrConfig(tolerance=1e-8, max_outer=30, inner_tolerance=1e-4,
-                        restart=30, max_inner=50000,
-                        inner_method=spk.IR_FGMRES,
-                        orthogonalization=spk.ORTHO_CGS2)
-    run("ir_fp32_fgmres30_learned", lambda x: spk.ir_solve(b, x, icfg, A, Ml))
-    res["speedup_learned_ir_vs_jacobi_pcg"] = round(
-        jt / res["ir_fp32_fgmres30_learned"]["ms"], 3)
-    res["speedup_learned_vs_untrained_fgmres"] = round(
-        res["fgmres30_untrained_jacobi_fp64"]["ms"] /
-        res["fgmres30_learned_fp64"]["ms"], 3)
+    # the untrained network (the Jacobi inverse) in the same slot: a bounded
+    # sample (it does not converge on this matrix; 50,000 iterations left
+    # it at 1.9e-5)
+    fgu = spk.FgmresConfig(1e-8, 30, 3000, False, spk.F64, spk.ORTHO_CGS2)
+    run("fgmres30_untrained_jacobi_fp64_3000_iterations",
+        lambda x: spk.fgmres(b, x, fgu, A, Mj), warm=False)
+    # the guarded solve: the same FGMRES behind the check of M and the stall
+    # monitor (it needs no fallback here), and with a poisoned network (NaN
+    # output layer), which falls back to Jacobi-PCG
+    lcfg = spk.LearnedConfig(tolerance=1e-8, restart=30, max_iterations=50000)
+    guard = {}
+
+    def guarded(name, M):
+        def fn(x):
+            guard[name] = spk.learned_solve(b, x, lcfg, A, M)
+            return guard[name]
+        run(name, fn)
+        g = guard[name]
+        res[name].update(method=g.method, reasons=g.reasons,
+                         learned_iterations=g.learned_iterations,
+                         fallback_iterations=g.fallback_iterations,
+                         converged=g.converged, true_relres=g.relres)
+
+    guarded("guarded_fgmres30_learned", Ml)
+    poisoned = spk.SpaiModel(1)
+    pp = model.params
+    pp[SPAI_W3:SPAI_B3] = np.nan
+    poisoned.params = pp
+    guarded("guarded_fgmres30_nan_network", poisoned.preconditioner(A))
+    # the metric's own ratio: learned-SPAI FGMRES against Jacobi-PCG, both
+    # to the 1e-8 FP64 true residual
+    res["learned_fgmres_vs_jacobi_pcg"] = speedup(
+        res, "jacobi_pcg_fp64", "fgmres30_learned_fp64")
+    res["fallback_overhead_vs_jacobi_pcg"] = speedup(
+        res, "jacobi_pcg_fp64", "guarded_fgmres30_nan_network")
     return res
 
 
@@ -526,19 +673,6 @@ def poisson2d_section(spk, side=256):
     return res
 
 
-def cpu_poisson2d(side=256):
-    """The oracle's full Jacobi-PCG solve of the same poisson_2d case."""
-    from oracle import oracle as O
-    A = O.poisson_2d(side, side, 0.0)
-    b = O.uniform_pm1(42, A.nrows)
-    b = b - b.mean()
-    t0 = time.perf_counter()
-    _, s = O.cg(A, b, tol=1e-8, restart_iterations=20000,
-                max_iterations=20000, left="jacobi")
-    return {"config1_full_solve_s": round(time.perf_counter() - t0, 3),
-            "config1_iterations": s.computed_iterations}
-
-
 def batched_section(spk, count=4096, seed=42, smin=32, smax=256,
                     sample=256, reps=3):
     """Batched Jacobi-PCG over `count` 2D variable-coefficient diffusion
@@ -611,7 +745,8 @@ def batched_section(spk, count=4096, seed=42, smin=32, smax=256,
     res["fp64_graph"] = {"ms": round(ms64, 2), "converged": s64["converged"],
                          "max_true_relres": s64["max_relres"],
                          "inner_iterations": s64["total_inner_iterations"]}
-    res["speedup_fp32_inner_vs_fp64"] = round(ms64 / ms32, 3)
+    both = s32["converged"] == count and s64["converged"] == count
+    res["speedup_fp32_inner_vs_fp64"] = round(ms64 / ms32, 3) if both else None
     log(f"batched: {count} systems n={n} nnz={nnz}: tuned {chosen} "
         f"({t_tune:.0f} ms); FP32-inner {ms32:.1f} ms "
         f"{s32['total_inner_iterations']} it ({gbs:.0f} GB/s), FP64 "
@@ -628,8 +763,13 @@ def main():
     ap.add_argument("--powerlaw-n", type=int, default=4194304)
     ap.add_argument("--inputs", default="ab", help="a, b or ab")
     ap.add_argument("--spmv-reps", type=int, default=30)
-    ap.add_argument("--cpu-iters", type=int, default=12)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-legs", default=",".join(CPU_LEGS),
+                    help="oracle legs timed on host cores beside the GPU "
+                         "work: " + ",".join(CPU_LEGS))
+    ap.add_argument("--cpu-deadline", type=float, default=480.0,
+                    help="seconds after start when unfinished legs stop")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plain CSR layout (no column panels)")
     ap.add_argument("--sections",
@@ -654,6 +794,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # the CPU legs start first, in child processes pinned to their own
+    # cores, and run while the GPU is measured
+    legs_state = None
+    if not args.no_cpu:
+        legs_state = start_cpu_legs([g for g in args.cpu_legs.split(",") if g])
     import sparkit_amd as spk
     spk.init(0)
     cus, name = spk.device_info()
@@ -810,11 +955,11 @@ def main():
         section("spai_ceiling", lambda: spai_ceiling_section(
             spk, n_side=args.spai_side, train_count=args.spai_train,
             epochs=args.spai_epochs))
-    if not args.no_cpu and "a" in args.inputs:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.grid, args.cpu_iters)
-        if "p2d" in secs:
-            out["cpu_baseline"].update(cpu_poisson2d())
+    if legs_state is not None:
+        log("cpu baseline: waiting for the legs ...")
+        legs = collect_cpu_legs(legs_state, args.cpu_deadline)
+        out["cpu_baseline"] = cpu_baseline(legs, legs_state, inputs)
+        attach_cpu_legs(out, legs)
     print(json.dumps(out), flush=True)
 
 

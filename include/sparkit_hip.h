@@ -272,7 +272,9 @@ spk_status spk_ilu0_download(spk_precond M, int64_t* l_row_ptr,
 spk_status spk_ilu0_levels(spk_precond M, int64_t* levels_l,
                            int64_t* levels_u);
 /* forward_solve(L, b) / backward_solve(U, b) with the factors of an ILU
- * preconditioner (triangular_solve.hpp:27-55, 67-98). */
+ * preconditioner (triangular_solve.hpp:27-55, 67-98).  The reference
+ * returns a new vector; here x is the caller's output and must not overlap
+ * b (SPK_INVALID_ARGUMENT): the sync-free sweeps mark x before reading b. */
 spk_status spk_ilu0_forward(spk_precond M, int dtype, const void* d_b,
                             void* d_x);
 spk_status spk_ilu0_backward(spk_precond M, int dtype, const void* d_b,
@@ -493,6 +495,74 @@ spk_status spk_spai_train(spk_spai_model m, const spk_csr* mats, int64_t count,
                           double* loss_history);
 /* M on A's pattern as a new store (the caller destroys it) */
 spk_status spk_spai_predict(spk_spai_model m, spk_csr A, spk_csr* M_out);
+
+/* ------------------------------------ learned solve with a fallback
+ * An explicit approximate inverse M (spk_precond_create_spmv, e.g. over
+ * spk_spai_predict's store) in FGMRES(m)'s preconditioner slot
+ * (fgmres.hpp:452-470; the slot is applied at fgmres.hpp:246-256), guarded,
+ * with the FP64 true residual as the judge:
+ *  1. M is checked before use: every value finite, every row's diagonal
+ *     entry present, finite and positive;
+ *  2. FGMRES runs in FP64 and is stopped early when `stall_cycles`
+ *     restart cycles in a row each fail to bring the residual estimate
+ *     below stall_factor times the previous cycle's, or when it turns
+ *     non-finite;
+ *  3. x is accepted when ||b - A x|| / ||b|| < tolerance (FP64).
+ * Otherwise the fallback solves from x0 (or from the learned x when that
+ * is finite and better than x0): Jacobi-PCG (conjugate_gradient.hpp:
+ * 122-312 with jacobi.hpp:25-50, restart_iterations = max_iterations) when A
+ * is symmetric, else GMRES(m) with ILU(0) in the right slot (gmres.hpp:
+ * 99-375, ilu.hpp:35-179) -- or the one `fallback` names.  M may be NULL
+ * (the fallback runs at once).  No reference counterpart: the reference has
+ * no learned preconditioner and does not detect a failing one. */
+enum {
+  SPK_FALLBACK_AUTO = 0,       /* Jacobi-PCG if A is symmetric, else ILU-GMRES */
+  SPK_FALLBACK_JACOBI_PCG = 1,
+  SPK_FALLBACK_ILU_GMRES = 2,
+  SPK_FALLBACK_NONE = 3        /* report the learned result only */
+};
+/* spk_learned_summary.method: the solver whose x is returned */
+enum {
+  SPK_METHOD_LEARNED_FGMRES = 0,
+  SPK_METHOD_JACOBI_PCG = 1,
+  SPK_METHOD_ILU_GMRES = 2
+};
+/* spk_learned_summary.reason: why the fallback ran (bits) */
+enum {
+  SPK_REASON_NO_PRECONDITIONER = 1, /* M was NULL */
+  SPK_REASON_BAD_VALUES = 2,        /* M holds NaN or infinite values */
+  SPK_REASON_BAD_DIAGONAL = 4,      /* a diagonal entry missing, <= 0 or not finite */
+  SPK_REASON_STALLED = 8,           /* FGMRES stopped making progress */
+  SPK_REASON_NOT_CONVERGED = 16     /* budget spent above the tolerance */
+};
+typedef struct {
+  double tolerance;                /* FP64 true relative residual target */
+  int64_t restart;                 /* FGMRES / fallback GMRES restart dimension */
+  int64_t max_iterations;          /* FGMRES iterations */
+  int64_t fallback_max_iterations; /* fallback solver iterations */
+  double stall_factor;             /* in (0, 1] */
+  int32_t stall_cycles;            /* >= 1 */
+  int32_t fallback;                /* SPK_FALLBACK_* */
+  int32_t orthogonalization;       /* FGMRES: SPK_ORTHO_MGS or SPK_ORTHO_CGS2 */
+  int32_t pad;
+} spk_learned_config;
+typedef struct {
+  int32_t method;              /* SPK_METHOD_* */
+  int32_t reason;              /* SPK_REASON_* bits, 0 if none */
+  int32_t converged;           /* relres < tolerance */
+  int32_t pad;
+  int64_t learned_iterations;  /* FGMRES iterations run */
+  int64_t fallback_iterations; /* fallback iterations run */
+  int64_t bad_values;          /* non-finite entries of M */
+  int64_t bad_diagonal;        /* rows of M with a bad diagonal */
+  double learned_relres;       /* FP64 true residual after FGMRES (NaN if not run) */
+  double relres;               /* FP64 true relative residual of the returned x */
+} spk_learned_summary;
+
+/* b, x: FP64 device vectors; x holds x0 on entry and the solution on exit */
+spk_status spk_learned_solve(spk_csr A, spk_precond M,
+                             const spk_learned_config* cfg, const double* d_b,
+                             double* d_x, spk_learned_summary* summary);
 
 /* ------------------------------------------------------- batched PCG
  * Thousands of independent SPD systems, stored as the diagonal blocks of

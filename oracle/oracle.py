@@ -411,12 +411,13 @@ class _Op:
             self.ctx = None
             return
         kind, data = spec if isinstance(spec, tuple) else (spec, None)
-        if kind in ("csr", "csrT"):
+        if kind in ("csr", "csrT", "csr_ref"):
             M = data if data is not None else A
             v = np.ascontiguousarray(M.vals, dtype)
             s = M._struct(v)
             self.keep += [v, s, M]
-            nm = "orc_apply_csr" if kind == "csr" else "orc_apply_csr_transpose"
+            nm = {"csr": "orc_apply_csr", "csr_ref": "orc_apply_csr_ref",
+                  "csrT": "orc_apply_csr_transpose"}[kind]
             self.fn = C.cast(getattr(L, f"{nm}_{sfx}"), C.c_void_p)
             self.ctx = C.cast(C.pointer(s), C.c_void_p)
         elif kind == "jacobi":
@@ -426,14 +427,15 @@ class _Op:
             self.fn = C.cast(getattr(L, f"orc_apply_jacobi_{sfx}"),
                              C.c_void_p)
             self.ctx = _ptr(inv_d)
-        elif kind in ("ilu", "milu"):
+        elif kind in ("ilu", "milu", "ilu_ref"):
             Lf, Uf = data if data is not None else ilu0(A, kind == "milu")
             lv = np.ascontiguousarray(Lf.vals, dtype)
             uv = np.ascontiguousarray(Uf.vals, dtype)
             tmp = np.zeros(A.nrows, dtype)
             s = IluD(Lf._struct(lv), Uf._struct(uv), _ptr(tmp))
             self.keep += [Lf, Uf, lv, uv, tmp, s]
-            self.fn = C.cast(getattr(L, f"orc_apply_ilu_{sfx}"), C.c_void_p)
+            nm = "orc_apply_ilu_ref" if kind == "ilu_ref" else "orc_apply_ilu"
+            self.fn = C.cast(getattr(L, f"{nm}_{sfx}"), C.c_void_p)
             self.ctx = C.cast(C.pointer(s), C.c_void_p)
         elif kind == "ilu_jac":
             # ("ilu_jac", k) or ("ilu_jac", (L, U, k))
@@ -450,8 +452,19 @@ class _Op:
             raise ValueError(f"unknown operator kind {kind!r}")
 
 
+def _ref_kind(spec):
+    """The reference-call-pattern counterpart of a slot (ILU: a fresh
+    vector per triangular solve); other slots allocate nothing in the
+    reference and are unchanged."""
+    if spec == "ilu":
+        return "ilu_ref"
+    if isinstance(spec, tuple) and spec[0] == "ilu":
+        return ("ilu_ref", spec[1])
+    return spec
+
+
 def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
-           dtype):
+           dtype, ref_calls=False):
     L = lib()
     dtype = np.dtype(dtype)
     sfx = "f" if dtype == np.float32 else "d"
@@ -463,7 +476,9 @@ def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
     # two-matvec steps (BiCGSTAB, CGS, BiCG, QMR) can run one matvec past an
     # odd budget, one history entry per matvec
     hist = np.zeros(max(max_iterations, 1) + 2, dtype)
-    a_op = _Op(("csr", A), A, dtype)
+    if ref_calls:  # the reference's per-call allocations and copies
+        left, right = _ref_kind(left), _ref_kind(right)
+    a_op = _Op(("csr_ref" if ref_calls else "csr", A), A, dtype)
     ml = _Op(left, A, dtype)
     if name in ("bicg", "qmr"):
         at = _Op(("csrT", A), A, dtype)
@@ -487,17 +502,19 @@ def _solve(name, A: Csr, b, x0, tol, restart, max_iterations, left, right,
 
 
 def cg(A, b, x0=None, *, tol, restart_iterations, max_iterations, left=None,
-       right=None, dtype=np.float64):
-    """conjugate_gradient(): conjugate_gradient.hpp:122-366."""
+       right=None, dtype=np.float64, ref_calls=False):
+    """conjugate_gradient(): conjugate_gradient.hpp:122-366.  ref_calls:
+    the operator slots allocate and copy per call as the reference's
+    multiply / ilu_apply do (same results; for timing)."""
     return _solve("cg", A, b, x0, tol, restart_iterations, max_iterations,
-                  left, right, dtype)
+                  left, right, dtype, ref_calls)
 
 
 def gmres(A, b, x0=None, *, tol, restart_dimension, max_iterations, left=None,
-          right=None, dtype=np.float64):
-    """gmres(): gmres.hpp:99-427."""
+          right=None, dtype=np.float64, ref_calls=False):
+    """gmres(): gmres.hpp:99-427 (ref_calls as for cg)."""
     return _solve("gmres", A, b, x0, tol, restart_dimension, max_iterations,
-                  left, right, dtype)
+                  left, right, dtype, ref_calls)
 
 
 def fgmres(A, b, x0=None, *, tol, restart_dimension, max_iterations,

@@ -306,6 +306,44 @@ ORC_TRISOLVE(f, float)
 ORC_OPS(d, double, orc_csr_d, orc_ilu_d)
 ORC_OPS(f, float, orc_csr_f, orc_ilu_f)
 
+/* The same operators with the reference's per-call memory traffic, for
+ * timing the CPU baseline like the reference runs (results are identical):
+ *   the operator lambda over multiply (conjugate_gradient_test.cpp:61-64):
+ *   multiply returns a new std::vector<T>(rows, T{0}) (sparse_blas.hpp:25),
+ *   which the lambda copies to out;
+ *   ilu_apply (ilu.hpp:173-179): forward_solve and backward_solve each
+ *   return a new vector initialised from their input (triangular_solve.hpp:
+ *   41, 81), and the result is copied to out. */
+#define ORC_OPS_REF(SFX, T, CSR, ILU)                                          \
+  void orc_apply_csr_ref_##SFX(const void* ctx, int64_t n, const T* in,        \
+                               T* out) {                                       \
+    const CSR* A = (const CSR*)ctx;                                            \
+    (void)n;                                                                   \
+    T* y = (T*)malloc((size_t)(A->nrows > 0 ? A->nrows : 1) * sizeof(T));      \
+    if (!y) abort();                                                           \
+    for (int64_t i = 0; i < A->nrows; ++i) y[i] = (T)0;                        \
+    orc_spmv_##SFX(A->nrows, A->row_ptr, A->col_ind, A->vals, in, y);          \
+    memcpy(out, y, (size_t)A->nrows * sizeof(T));                              \
+    free(y);                                                                   \
+  }                                                                            \
+  void orc_apply_ilu_ref_##SFX(const void* ctx, int64_t n, const T* in,        \
+                               T* out) {                                       \
+    const ILU* F = (const ILU*)ctx;                                            \
+    const size_t bytes = (size_t)(n > 0 ? n : 1) * sizeof(T);                  \
+    T* y = (T*)malloc(bytes);                                                  \
+    if (!y) abort();                                                           \
+    orc_forward_solve_##SFX(n, F->L.row_ptr, F->L.col_ind, F->L.vals, in, y);  \
+    T* z = (T*)malloc(bytes);                                                  \
+    if (!z) abort();                                                           \
+    orc_backward_solve_##SFX(n, F->U.row_ptr, F->U.col_ind, F->U.vals, y, z);  \
+    free(y);                                                                   \
+    memcpy(out, z, (size_t)n * sizeof(T));                                     \
+    free(z);                                                                   \
+  }
+
+ORC_OPS_REF(d, double, orc_csr_d, orc_ilu_d)
+ORC_OPS_REF(f, float, orc_csr_f, orc_ilu_f)
+
 /* Approximate ilu_apply (no reference counterpart; it replaces the exact
  * triangular solves of ilu.hpp:173-179 / triangular_solve.hpp:27-98 by k
  * Jacobi iterations each):

@@ -179,6 +179,13 @@ void orc_apply_identity_f(const void* ctx, int64_t n, const float* in, float* ou
 /* operator lambda over multiply(A, x): conjugate_gradient_test.cpp:61-64 */
 void orc_apply_csr_d(const void* ctx, int64_t n, const double* in, double* out);
 void orc_apply_csr_f(const void* ctx, int64_t n, const float* in, float* out);
+/* the same two operators with the reference's per-call allocation and
+ * copy (sparse_blas.hpp:25, ilu.hpp:173-179, triangular_solve.hpp:41,81);
+ * identical results, used to time the CPU baseline */
+void orc_apply_csr_ref_d(const void* ctx, int64_t n, const double* in, double* out);
+void orc_apply_csr_ref_f(const void* ctx, int64_t n, const float* in, float* out);
+void orc_apply_ilu_ref_d(const void* ctx, int64_t n, const double* in, double* out);
+void orc_apply_ilu_ref_f(const void* ctx, int64_t n, const float* in, float* out);
 /* multiply_transpose(A, x): sparse_blas.hpp:36-52 */
 void orc_spmv_transpose_d(int64_t nrows, int64_t ncols, const int64_t* rp,
                           const int64_t* ci, const double* v, const double* x,

@@ -38,7 +38,9 @@ __all__ = [
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
-    "true_relres",
+    "true_relres", "dot", "axpby", "learned_solve", "LearnedConfig",
+    "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
+    "FALLBACK_ILU_GMRES", "FALLBACK_NONE",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB", "IR_CG_RELIABLE",
@@ -811,6 +813,88 @@ def ir_solve(b: DeviceVector, x: DeviceVector, cfg: IrConfig, A: CsrMatrix,
          hist.ctypes.data_as(C.c_void_p))
     return IrSummary(s.relres, s.outer_iterations, s.inner_iterations,
                      bool(s.converged), hist[:s.outer_iterations + 1].tolist())
+
+
+FALLBACK_AUTO, FALLBACK_JACOBI_PCG, FALLBACK_ILU_GMRES, FALLBACK_NONE = 0, 1, 2, 3
+METHODS = {0: "learned_fgmres", 1: "jacobi_pcg", 2: "ilu_gmres"}
+REASONS = {1: "no_preconditioner", 2: "bad_values", 4: "bad_diagonal",
+           8: "stalled", 16: "not_converged"}
+
+
+@dataclass
+class LearnedConfig:
+    """spk_learned_config: FGMRES(restart) with the learned M, guarded; the
+    fallback (Jacobi-PCG for symmetric A, else ILU(0)-GMRES) when M is bad,
+    FGMRES stalls or the FP64 true residual misses the tolerance."""
+    tolerance: float = 1e-8
+    restart: int = 30
+    max_iterations: int = 50000
+    fallback_max_iterations: int = 50000
+    stall_factor: float = 0.999
+    stall_cycles: int = 20
+    fallback: int = FALLBACK_AUTO
+    orthogonalization: int = ORTHO_CGS2
+
+
+@dataclass
+class LearnedSummary:
+    method: str
+    reasons: list
+    converged: bool
+    learned_iterations: int
+    fallback_iterations: int
+    bad_values: int
+    bad_diagonal: int
+    learned_relres: float
+    relres: float
+
+    @property
+    def fell_back(self) -> bool:
+        return self.method != "learned_fgmres"
+
+    @property
+    def computed_iterations(self) -> int:
+        """FGMRES and fallback iterations together."""
+        return self.learned_iterations + self.fallback_iterations
+
+
+def learned_solve(b: DeviceVector, x: DeviceVector, cfg: LearnedConfig,
+                  A: CsrMatrix, M=None) -> LearnedSummary:
+    """x (FP64, in place) by FGMRES with the explicit approximate inverse M
+    (an SpMV preconditioner, e.g. SpaiModel.preconditioner(A)) and the
+    fallback; the FP64 true residual decides."""
+    if b.dtype != np.float64 or x.dtype != np.float64:
+        raise InvalidArgument(1, "learned_solve", "b and x must be float64")
+    c = _lib.LearnedConfig(float(cfg.tolerance), int(cfg.restart),
+                           int(cfg.max_iterations),
+                           int(cfg.fallback_max_iterations),
+                           float(cfg.stall_factor), int(cfg.stall_cycles),
+                           int(cfg.fallback), int(cfg.orthogonalization), 0)
+    s = _lib.LearnedSummary()
+    call("spk_learned_solve", A.h, _h(M), C.byref(c), b.ptr, x.ptr,
+         C.byref(s))
+    return LearnedSummary(
+        METHODS[s.method], [v for k, v in REASONS.items() if s.reason & k],
+        bool(s.converged), s.learned_iterations, s.fallback_iterations,
+        s.bad_values, s.bad_diagonal, s.learned_relres, s.relres)
+
+
+def dot(x: DeviceVector, y: DeviceVector) -> float:
+    """<x, y> summed in FP64 on the device, in a fixed order (the
+    std::inner_product of conjugate_gradient.hpp:194,236; the order of the
+    sum differs, so results agree to rounding, not bit for bit)."""
+    if x.n != y.n or x.dtype != y.dtype:
+        raise InvalidArgument(1, "dot", "x and y differ in length or dtype")
+    r = C.c_double()
+    call("spk_dot", _code(x.dtype), x.n, x.ptr, y.ptr, C.byref(r))
+    return r.value
+
+
+def axpby(a: float, x: DeviceVector, b: float, y: DeviceVector):
+    """y = a x + b y in the vectors' precision, in place."""
+    if x.n != y.n or x.dtype != y.dtype:
+        raise InvalidArgument(1, "axpby", "x and y differ in length or dtype")
+    call("spk_axpby", _code(x.dtype), x.n, float(a), x.ptr, float(b), y.ptr)
 
 
 def true_relres(A: CsrMatrix, b: DeviceVector, x: DeviceVector) -> float:

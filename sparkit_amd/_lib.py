@@ -94,6 +94,24 @@ class IrSummary(C.Structure):
                 ("status", C.c_int32)]
 
 
+class LearnedConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("restart", C.c_int64),
+                ("max_iterations", C.c_int64),
+                ("fallback_max_iterations", C.c_int64),
+                ("stall_factor", C.c_double), ("stall_cycles", C.c_int32),
+                ("fallback", C.c_int32), ("orthogonalization", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class LearnedSummary(C.Structure):
+    _fields_ = [("method", C.c_int32), ("reason", C.c_int32),
+                ("converged", C.c_int32), ("pad", C.c_int32),
+                ("learned_iterations", C.c_int64),
+                ("fallback_iterations", C.c_int64),
+                ("bad_values", C.c_int64), ("bad_diagonal", C.c_int64),
+                ("learned_relres", C.c_double), ("relres", C.c_double)]
+
+
 vp = C.c_void_p
 i64 = C.c_int64
 u64 = C.c_uint64
@@ -138,6 +156,7 @@ _SIGS = {
     "spk_csr_get_panels": ([vp, P_i64], None),
     "spk_csr_tune_layout": ([vp, P_i64, vp, vp, P_i64], None),
     "spk_dot": ([i32, i64, vp, vp, P_dbl], None),
+    "spk_learned_solve": ([vp, vp, vp, vp, vp, vp], None),
     "spk_axpby": ([i32, i64, dbl, vp, dbl, vp], None),
     "spk_convert": ([i64, i32, vp, i32, vp], None),
     "spk_precond_create_jacobi": ([vp, C.POINTER(vp)], None),

@@ -7,7 +7,9 @@
 // Breakdowns the reference leaves undetected stop the solve with status
 // SPK_BREAKDOWN: rho_new == 0 (the reference returns without consuming
 // budget and loops forever, bicgstab.hpp:203-204) and omega == 0 (it would
-// divide by omega in the next beta, :206, :222-223).
+// divide by omega in the next beta, :206, :222-223).  On omega == 0 the
+// step's valid half (x += alpha p^, r = s) is kept before stopping, so the
+// returned x satisfies the reported residual norm |s|.
 #include "spk_solver_common.hpp"
 
 namespace spk {
@@ -184,7 +186,7 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         n, pq[3], c_r, (const double*)nullptr, 0, BiGateNotDone{done},
         [=] __device__(const double* t, double* sc, bool b0) {
           const double rn2 = t[0], rho_new = t[1];
-          bool stop = false;
+          bool stop = S->breakdown != 0;  // omega == 0 in the last step
           if (test_r && sqrt(rn2) / S->bnorm < S->tol) stop = true;
           const double beta = (rho_new / S->rho) * (S->alpha / S->omega);
           sc[0] = beta;
@@ -192,6 +194,7 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
           sc[2] = (stop || rho_new == 0.0) ? 1.0 : 0.0;
           if (b0) {
             if (test_r) bi_r_test(S, rn2);
+            if (S->breakdown) S->done = 1;
             if (!stop) {
               S->rho_new = rho_new;
               if (rho_new == 0.0) {
@@ -280,16 +283,14 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
             S->iters += 1;
             S->omega = om;
             S->rho = S->rho_new;
-            if (om == 0.0) {
-              S->breakdown = 1;
-              S->done = 1;
-            }
+            if (om == 0.0) S->breakdown = 1;
           }
         },
         [=] __device__(int64_t i) { return Pack8<T>{x[i], php[i], shp[i], sv[i], t[i],
                                                     rh[i], T(0), T(0)}; },
         [=] __device__(int64_t i, const Pack8<T>& e, double* acc, const double* sc) {
-          if (sc[1] == 0.0) return;
+          // omega == 0 (breakdown) makes this the half step: x += alpha p^,
+          // r = s; the next K1 (or the tail) tests |r| and ends the solve
           x[i] = (T)((double)e.a + (sc[0] * (double)e.b + sc[1] * (double)e.c));
           const T ri = (T)((double)e.d - sc[1] * (double)e.e);
           r[i] = ri;
@@ -354,10 +355,9 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
       S->iters += 1;
       double om = tt[0] == 0.0 ? 0.0 : tt[1] / tt[0];
       S->omega = om;
-      if (om == 0.0) {
-        S->breakdown = 1;
-        S->done = 1;
-      }
+      // omega == 0: the update below still runs (with omega = 0 it is the
+      // half step x += alpha p^, r = s) and its finisher ends the solve
+      if (om == 0.0) S->breakdown = 1;
     }, done);
     // x += alpha p^ + omega s^; r = s - omega t; |r| (:299-324)
     launch_reduce<1>(n, [=] __device__(int64_t i, double* acc) {
@@ -374,6 +374,7 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
         S->converged = 1;
         S->done = 1;
       }
+      if (S->breakdown) S->done = 1;
     }, done);
   };
   // each step consumes two matvecs; steps run while budget remains (:176)
@@ -385,7 +386,10 @@ void bicgstab_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
     launch_reduce_pro<2, 0, 0>(
         1, pq[3], c_r, (const double*)nullptr, 0, BiGateNotDone{done},
         [=] __device__(const double* t, double*, bool b0) {
-          if (b0) bi_r_test(S, t[0]);
+          if (b0) {
+            bi_r_test(S, t[0]);
+            if (S->breakdown) S->done = 1;
+          }
         },
         [=] __device__(int64_t) { return Pack4<T>{T(0), T(0), T(0), T(0)}; },
         [=] __device__(int64_t, const Pack4<T>&, double*, const double*) {},

@@ -254,6 +254,10 @@ spk_status spk_dot(int dtype, int64_t n, const void* d_x, const void* d_y,
                    double* result) {
   return guarded([&] {
     SPK_REQUIRE(result && n >= 0, SPK_INVALID_ARGUMENT, "bad argument");
+    SPK_REQUIRE(dtype == SPK_F32 || dtype == SPK_F64, SPK_INVALID_ARGUMENT,
+                "dtype must be SPK_F32 or SPK_F64");
+    SPK_REQUIRE(n == 0 || (d_x && d_y), SPK_INVALID_ARGUMENT,
+                "null vector pointer");
     *result = dtype == SPK_F32 ? dot_impl(n, (const float*)d_x, (const float*)d_y)
                                : dot_impl(n, (const double*)d_x, (const double*)d_y);
   });
@@ -262,6 +266,11 @@ spk_status spk_dot(int dtype, int64_t n, const void* d_x, const void* d_y,
 spk_status spk_axpby(int dtype, int64_t n, double a, const void* d_x, double b,
                      void* d_y) {
   return guarded([&] {
+    SPK_REQUIRE(n >= 0, SPK_INVALID_ARGUMENT, "bad argument");
+    SPK_REQUIRE(dtype == SPK_F32 || dtype == SPK_F64, SPK_INVALID_ARGUMENT,
+                "dtype must be SPK_F32 or SPK_F64");
+    SPK_REQUIRE(n == 0 || (d_x && d_y), SPK_INVALID_ARGUMENT,
+                "null vector pointer");
     if (dtype == SPK_F32) {
       const float* x = (const float*)d_x;
       float* y = (float*)d_y;

@@ -170,7 +170,8 @@ __global__ void back_substitute_kernel(GmState* st) {
 template <typename T>
 void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
                  spk_precond_s* MR, const spk_solver_config* cfg, const T* b,
-                 T* x, spk_summary* out, double* history) {
+                 T* x, spk_summary* out, double* history,
+                 const CycleMonitor* monitor = nullptr) {
   SPK_REQUIRE(cfg->restart > 0, SPK_INVALID_ARGUMENT,
               "restart_dimension must be > 0");
   const int64_t n = A->nrows, m = cfg->restart, ld = m + 1;
@@ -363,6 +364,8 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
     }, done);
     h = st.download();
     iters = h.iters;
+    if (monitor && !h.converged && !h.done && !(*monitor)(iters, h.resnorm))
+      break;
   }
   out->residual_norm = h.resnorm;
   out->computed_iterations = h.iters;
@@ -377,6 +380,14 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
 }
 
 }  // namespace
+
+void fgmres_monitored(spk_csr_s* A, spk_precond_s* M,
+                      const spk_solver_config* cfg, const double* b, double* x,
+                      spk_summary* out, const CycleMonitor& monitor) {
+  *out = spk_summary{};
+  gmres_solve<double>(true, A, M, nullptr, cfg, b, x, out, nullptr, &monitor);
+}
+
 }  // namespace spk
 
 using namespace spk;

@@ -343,13 +343,19 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   restart(true);
   int64_t k = 0;
   if (history) history[0] = rnorm / bnorm;
+  // `issued` numbers the queued steps (parity and step stamps); the budget
+  // is charged with the steps the device ran (h.iters): steps queued after
+  // the device raised `hold` are no-ops and cost nothing
+  int64_t ran = 0;
   for (;;) {
     // FP32 steps until an update is due or the budget is spent
     int64_t chunk = 4;
-    while (issued < cfg->max_inner) {
-      const int64_t e = std::min(cfg->max_inner, issued + chunk);
-      for (; issued < e; ++issued) step();
-      if (st.download().hold) break;
+    while (ran < cfg->max_inner) {
+      const int64_t e = std::min(cfg->max_inner - ran, chunk);
+      for (int64_t j = 0; j < e; ++j, ++issued) step();
+      h = st.download();
+      ran = h.iters;
+      if (h.hold) break;
       chunk = std::min<int64_t>(chunk * 2, 64);
     }
     // reliable update: x64 += xi, FP64 true residual, convergence test
@@ -364,7 +370,7 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
     h = st.download();
     ++k;
     if (history) history[k] = h.rnorm / h.bnorm;
-    if (h.converged || k >= cfg->max_outer || issued >= cfg->max_inner) break;
+    if (h.converged || k >= cfg->max_outer || h.iters >= cfg->max_inner) break;
     restart(false);
   }
   out->relres = h.rnorm / h.bnorm;

@@ -879,11 +879,23 @@ spk_status spk_ilu0_levels(spk_precond M, int64_t* levels_l,
   });
 }
 
+// The sync-free sweeps mark x with a sentinel before they read b, so an
+// input that overlaps the output would be overwritten first.
+static bool vectors_overlap(const void* b, const void* x, int64_t n, int dtype) {
+  const size_t bytes = (size_t)(n > 0 ? n : 0) * (dtype == SPK_F32 ? 4 : 8);
+  const uintptr_t pb = (uintptr_t)b, px = (uintptr_t)x;
+  return pb == px || (pb < px + bytes && px < pb + bytes);
+}
+
 spk_status spk_ilu0_forward(spk_precond M, int dtype, const void* d_b,
                             void* d_x) {
   return guarded([&] {
     SPK_REQUIRE(M && M->kind == spk_precond_s::kIlu && d_b && d_x,
                 SPK_INVALID_ARGUMENT, "bad argument");
+    SPK_REQUIRE(dtype == SPK_F32 || dtype == SPK_F64, SPK_INVALID_ARGUMENT,
+                "dtype must be SPK_F32 or SPK_F64");
+    SPK_REQUIRE(!vectors_overlap(d_b, d_x, M->n, dtype), SPK_INVALID_ARGUMENT,
+                "ilu0 triangular sweep: b and x must not overlap");
     if (dtype == SPK_F32) M->forward((const float*)d_b, (float*)d_x);
     else M->forward((const double*)d_b, (double*)d_x);
     M->check_sweeps();
@@ -895,6 +907,10 @@ spk_status spk_ilu0_backward(spk_precond M, int dtype, const void* d_b,
   return guarded([&] {
     SPK_REQUIRE(M && M->kind == spk_precond_s::kIlu && d_b && d_x,
                 SPK_INVALID_ARGUMENT, "bad argument");
+    SPK_REQUIRE(dtype == SPK_F32 || dtype == SPK_F64, SPK_INVALID_ARGUMENT,
+                "dtype must be SPK_F32 or SPK_F64");
+    SPK_REQUIRE(!vectors_overlap(d_b, d_x, M->n, dtype), SPK_INVALID_ARGUMENT,
+                "ilu0 triangular sweep: b and x must not overlap");
     if (dtype == SPK_F32) M->backward((const float*)d_b, (float*)d_x);
     else M->backward((const double*)d_b, (double*)d_x);
     M->check_sweeps();

@@ -2,10 +2,22 @@
 // workspace slot map, argument checks and the chunked issue loop.
 #pragma once
 
+#include <functional>
+
 #include "spk_precond.hpp"
 #include "spk_vec.hpp"
 
 namespace spk {
+
+// Host-side monitor of a restarted (F)GMRES solve: called after every
+// restart cycle that did not converge, with the iterations so far and the
+// residual estimate at the cycle's end; returning false ends the solve.
+using CycleMonitor = std::function<bool(int64_t iters, double resnorm)>;
+
+// FP64 FGMRES(m) (spk_fgmres_solve) under a cycle monitor.
+void fgmres_monitored(spk_csr_s* A, spk_precond_s* M,
+                      const spk_solver_config* cfg, const double* b, double* x,
+                      spk_summary* out, const CycleMonitor& monitor);
 
 // Workspace slots (each solver owns a range so vectors stay allocated
 // between calls and the timed region performs no hipMalloc).

@@ -136,6 +136,12 @@ def test_ilu0_schedules_bit_exact(O, spk, schedule):
         w = spk.DeviceVector.zeros(A.nrows)
         F.backward(y, w)
         assert np.array_equal(w.numpy(), z_ref)
+        # the standalone sweeps write a separate output: an overlapping
+        # call is rejected before anything is launched, on both schedules
+        for sweep in (F.forward, F.backward):
+            with pytest.raises(spk.InvalidArgument, match="overlap"):
+                sweep(y, y)
+        assert np.array_equal(y.numpy(), O.forward_solve(Lr, r))
         # switching schedules on a used preconditioner keeps it exact
         F.schedule = "sync-free" if schedule == "levels" else "levels"
         z2 = spk.DeviceVector.zeros(A.nrows)

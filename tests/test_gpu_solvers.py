@@ -169,6 +169,38 @@ def test_bicgstab_breakdown_reported(O, spk):
     assert np.array_equal(x, xt)
 
 
+@pytest.mark.parametrize("copies", [1, 1_100_000])  # hand-off / finisher path
+def test_bicgstab_omega_breakdown_keeps_half_step(spk, copies):
+    """omega = <t,s>/<t,t> = 0 exactly: copies of the 2x2 block
+    [[1, 1], [-1, 0]] with b = (1, 0) give alpha = 1, s = (0, 1) and
+    t = A s = (1, 0), so <t, s> = 0 in the first step.  The reference
+    returns before updating x and then divides by omega (bicgstab.hpp:222,
+    :206); the device solver keeps the valid half step (x += alpha p^,
+    r = s), stops with a breakdown, and reports |s|, which the returned x
+    satisfies.  1.1M copies (2.2M rows, > 4096 SpMV row blocks) take the
+    finisher path, one copy the one-kernel hand-off."""
+    n = 2 * copies
+    rp = np.zeros(n + 1, np.int64)
+    rp[1::2] = np.arange(copies) * 3 + 2
+    rp[2::2] = np.arange(1, copies + 1) * 3
+    base = np.arange(copies) * 2
+    ci = np.stack([base, base + 1, base], axis=1).ravel()
+    vals = np.tile([1.0, 1.0, -1.0], copies)
+    A = spk.CsrMatrix.from_host(n, n, rp, ci, vals)
+    b = np.tile([1.0, 0.0], copies)
+    x = spk.DeviceVector.zeros(n)
+    s = spk.bicgstab(spk.DeviceVector.from_numpy(b), x,
+                     spk.BicgstabConfig(1e-10, 100), A)
+    assert s.breakdown and not s.converged
+    assert s.computed_iterations == 2
+    xh = x.numpy()
+    assert np.array_equal(xh, np.tile([1.0, 0.0], copies))
+    x0, x1 = xh[0::2], xh[1::2]
+    r = np.stack([1.0 - (x0 + x1), x0], axis=1).ravel()  # b - A x per block
+    assert np.array_equal(r, np.tile([0.0, 1.0], copies))
+    assert s.residual_norm == np.sqrt(float(copies))  # |s| = |b - A x|
+
+
 def test_gmres_exact_start(O, spk):
     """beta = 0 at a restart (x0 exact) is reported as converged instead of
     the reference's division by zero (gmres.hpp:229-231)."""

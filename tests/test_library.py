@@ -49,3 +49,27 @@ def test_gfx950_code_object():
     """The library carries gfx950 device code."""
     data = open(LIB, "rb").read()
     assert b"gfx950" in data
+
+
+def test_blas1_entry_points_reject_bad_arguments():
+    """spk_dot / spk_axpby validate their arguments before any device call:
+    an unknown dtype, a negative length and null vectors with n > 0 return
+    SPK_INVALID_ARGUMENT (no GPU needed)."""
+    import sparkit_amd
+    from sparkit_amd import _lib as L
+    lib = sparkit_amd.load()
+    r = ctypes.c_double()
+    buf = ctypes.create_string_buffer(64)  # a non-null stand-in, never read
+    vp = ctypes.cast(buf, ctypes.c_void_p)
+    assert lib.spk_dot(7, 4, vp, vp, ctypes.byref(r)) == L.INVALID_ARGUMENT
+    assert b"dtype" in lib.spk_last_error()
+    assert lib.spk_dot(L.F64, -1, vp, vp, ctypes.byref(r)) == L.INVALID_ARGUMENT
+    assert lib.spk_dot(L.F64, 4, None, vp, ctypes.byref(r)) == L.INVALID_ARGUMENT
+    assert lib.spk_dot(L.F32, 4, vp, None, ctypes.byref(r)) == L.INVALID_ARGUMENT
+    assert lib.spk_dot(L.F64, 4, vp, vp, None) == L.INVALID_ARGUMENT
+    assert lib.spk_axpby(3, 4, 1.0, vp, 1.0, vp) == L.INVALID_ARGUMENT
+    assert b"dtype" in lib.spk_last_error()
+    assert lib.spk_axpby(L.F64, -2, 1.0, vp, 1.0, vp) == L.INVALID_ARGUMENT
+    assert lib.spk_axpby(L.F64, 4, 1.0, None, 1.0, vp) == L.INVALID_ARGUMENT
+    assert lib.spk_axpby(L.F32, 4, 1.0, vp, 1.0, None) == L.INVALID_ARGUMENT
+    assert b"null" in lib.spk_last_error()

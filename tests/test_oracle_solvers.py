@@ -249,3 +249,26 @@ def test_bicgstab_odd_budget_runs_a_whole_step(O, budget):
     assert s.computed_iterations == budget + 1
     assert len(s.iteration_residuals) == budget + 1
     assert np.all(np.isfinite(s.iteration_residuals))
+
+
+def test_reference_call_pattern_gives_identical_results(O):
+    """The timing variant of the operator slots (a fresh, zero-filled result
+    vector per multiply and per triangular solve, copied to out, as
+    sparse_blas.hpp:25 and ilu.hpp:173-179 do) computes the same numbers:
+    iterates and histories are bit-identical."""
+    A = O.convdiff_3d(9, 8, 7, 1.0, 0.3, 0.2, 0.1)
+    b = O.uniform_pm1(5, A.nrows)
+    for right in (None, "ilu"):
+        x1, s1 = O.gmres(A, b, tol=1e-10, restart_dimension=20,
+                         max_iterations=400, right=right)
+        x2, s2 = O.gmres(A, b, tol=1e-10, restart_dimension=20,
+                         max_iterations=400, right=right, ref_calls=True)
+        assert np.array_equal(x1, x2)
+        assert np.array_equal(s1.iteration_residuals, s2.iteration_residuals)
+    P = O.poisson_2d(20, 20, 0.1)
+    c = O.uniform_pm1(6, P.nrows)
+    x1, s1 = O.cg(P, c, tol=1e-10, restart_iterations=500, max_iterations=500,
+                  left="jacobi")
+    x2, s2 = O.cg(P, c, tol=1e-10, restart_iterations=500, max_iterations=500,
+                  left="jacobi", ref_calls=True)
+    assert np.array_equal(x1, x2) and s1.computed_iterations == s2.computed_iterations
