@@ -320,6 +320,24 @@ typedef struct {
 spk_status spk_cg_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
                         const spk_solver_config* cfg, const void* d_b,
                         void* d_x, spk_summary* summary, double* history);
+/* spk_cg_solve with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as spk_cg_solve, but every work vector, the solver state and
+ * the reduction partials live in one static 64 MiB device array of the
+ * library (about 1.5M rows in FP64, 3M in FP32; larger solves, and
+ * preconditioners other than identity, Jacobi or an explicit approximate
+ * inverse, return SPK_INVALID_ARGUMENT before any work is queued).  What the
+ * solve would otherwise make lazily -- a store's FP32 copy, a Jacobi
+ * preconditioner's FP32 copy, a column-panel store's running sums -- must
+ * already exist (one ordinary FP32 solve or SpMV makes them).  All calls on
+ * one GPU share that GPU's array: calls that overlap on one GPU (other
+ * threads, other streams) race and must be serialised by the caller. */
+spk_status spk_cg_solve_noalloc(spk_csr A, spk_precond M_left,
+                                spk_precond M_right,
+                                const spk_solver_config* cfg, const void* d_b,
+                                void* d_x, spk_summary* summary,
+                                double* history);
 /* gmres(b, x, cfg, A, M_L, M_R) (gmres.hpp:400-427), modified Gram-Schmidt
  * and Givens rotations as the reference. */
 spk_status spk_gmres_solve(spk_csr A, spk_precond M_left, spk_precond M_right,

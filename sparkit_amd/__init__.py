@@ -36,7 +36,7 @@ __all__ = [
     "identity", "jacobi", "ilu0", "milu0", "spai", "multiply",
     "extract_diagonal", "jacobi_apply", "ilu_apply", "CGConfig",
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
-    "conjugate_gradient", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
+    "conjugate_gradient", "conjugate_gradient_noalloc", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
     "true_relres", "dot", "axpby", "learned_solve", "LearnedConfig",
     "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
@@ -697,6 +697,17 @@ def conjugate_gradient(b, x, cfg: CGConfig, A: CsrMatrix, M_left=identity,
     c = _cfg(cfg, cfg.restart_iterations)
     return _run("spk_cg_solve", (A.h, _h(M_left), _h(M_right), C.byref(c),
                                  b.ptr, x.ptr), cfg)
+
+
+def conjugate_gradient_noalloc(b, x, cfg: CGConfig, A: CsrMatrix,
+                               M_left=identity, M_right=identity) -> Summary:
+    """conjugate_gradient() through spk_cg_solve_noalloc: the same solve,
+    with every work buffer in the library's static device scratch (no
+    device allocation; for callers that cannot allocate)."""
+    _vec_check(A, b, x, cfg.precision)
+    c = _cfg(cfg, cfg.restart_iterations)
+    return _run("spk_cg_solve_noalloc", (A.h, _h(M_left), _h(M_right),
+                                         C.byref(c), b.ptr, x.ptr), cfg)
 
 
 def gmres(b, x, cfg: GmresConfig, A: CsrMatrix, M_left=identity,

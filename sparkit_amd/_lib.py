@@ -176,6 +176,8 @@ _SIGS = {
     "spk_ilu0_backward": ([vp, i32, vp, vp], None),
     "spk_cg_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                       C.POINTER(Summary), vp], None),
+    "spk_cg_solve_noalloc": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
+                              C.POINTER(Summary), vp], None),
     "spk_gmres_solve": ([vp, vp, vp, C.POINTER(SolverConfig), vp, vp,
                          C.POINTER(Summary), vp], None),
     "spk_fgmres_solve": ([vp, vp, C.POINTER(SolverConfig), vp, vp,

@@ -453,3 +453,92 @@ extern "C" spk_status spk_cg_solve(spk_csr A, spk_precond M_left,
     check_preconds(M_left, M_right);
   });
 }
+
+namespace spk {
+namespace {
+
+// The no-allocation solve may only touch what already exists: a store's
+// lazily made FP32 copy and its column panels' running-sum vector.
+void require_ready(spk_csr_s& S, bool fp32) {
+  const char* msg32 =
+      "no-allocation solve: a store's FP32 copy is not made yet (run one FP32 "
+      "solve or SpMV with it first)";
+  const char* msgp =
+      "no-allocation solve: a column-panel store's running-sum vector is not "
+      "made yet (run one SpMV with it first)";
+  SPK_REQUIRE(!fp32 || S.has32, SPK_INVALID_ARGUMENT, msg32);
+  if (!S.panels.empty()) {
+    SPK_REQUIRE(fp32 ? S.pacc32.n >= (size_t)S.nrows : S.pacc64.n >= (size_t)S.nrows,
+                SPK_INVALID_ARGUMENT, msgp);
+    for (spk_csr_s* q : S.panels) SPK_REQUIRE(!fp32 || q->has32, SPK_INVALID_ARGUMENT, msg32);
+  }
+}
+
+size_t max_blocks(const spk_csr_s& S) {
+  size_t m = (size_t)S.nblocks;
+  for (const spk_csr_s* q : S.panels) m = std::max(m, (size_t)q->nblocks);
+  return m;
+}
+
+void require_noalloc_precond(spk_precond_s* M, bool fp32, size_t* blocks) {
+  if (is_identity(M)) return;
+  SPK_REQUIRE(M->kind == spk_precond_s::kJacobi || M->kind == spk_precond_s::kSpmv,
+              SPK_INVALID_ARGUMENT,
+              "no-allocation solve: preconditioners are identity, Jacobi or an "
+              "explicit approximate inverse (ILU(0) keeps lazily made buffers)");
+  if (M->kind == spk_precond_s::kJacobi) {
+    SPK_REQUIRE(!fp32 || M->has32, SPK_INVALID_ARGUMENT,
+                "no-allocation solve: the Jacobi preconditioner's FP32 copy is "
+                "not made yet (run one FP32 solve with it first)");
+  } else {
+    require_ready(*M->M, fp32);
+    *blocks = std::max(*blocks, max_blocks(*M->M));
+  }
+}
+
+}  // namespace
+}  // namespace spk
+
+extern "C" spk_status spk_cg_solve_noalloc(spk_csr A, spk_precond M_left,
+                                           spk_precond M_right,
+                                           const spk_solver_config* cfg,
+                                           const void* d_b, void* d_x,
+                                           spk_summary* summary,
+                                           double* history) {
+  return guarded([&] {
+    check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
+    SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
+    SPK_REQUIRE(cfg->restart > 0, SPK_INVALID_ARGUMENT,
+                "restart_iterations must be > 0 (the reference divides by it)");
+    *summary = spk_summary{};
+    const bool fp32 = cfg->precision == SPK_F32;
+    require_ready(*A, fp32);
+    size_t blocks = max_blocks(*A);
+    require_noalloc_precond(M_left, fp32, &blocks);
+    require_noalloc_precond(M_right, fp32, &blocks);
+    // what the solve carves: five n-vectors, the residual history, the
+    // state, three hand-off partial buffers and the reduction partials
+    const int64_t n = A->nrows;
+    const size_t vb = arena_round((size_t)(n > 0 ? n : 1) * (fp32 ? 4 : 8));
+    const size_t partials =
+        std::max(blocks, (size_t)2 * (size_t)vec_grid(n > 0 ? n : 1));
+    size_t need = 5 * vb + arena_round(sizeof(double) * partials) +
+                  3 * arena_round(sizeof(double) * 2 * (size_t)kProMaxPartials) +
+                  arena_round(1024);
+    if (cfg->collect_residuals)
+      need += arena_round(sizeof(double) *
+                          (size_t)std::max<int64_t>(cfg->max_iterations, 1));
+    SPK_REQUIRE(need <= scratch_bytes(), SPK_INVALID_ARGUMENT,
+                "no-allocation solve: needs " + std::to_string(need) +
+                    " bytes of scratch, more than its " +
+                    std::to_string(scratch_bytes()) +
+                    " (about 1.5M rows in FP64, 3M in FP32)");
+    ArenaScope arena(scratch_ptr(), scratch_bytes(), partials);
+    if (fp32)
+      cg_solve<float>(A, M_left, M_right, cfg, (const float*)d_b, (float*)d_x,
+                      summary, history);
+    else
+      cg_solve<double>(A, M_left, M_right, cfg, (const double*)d_b,
+                       (double*)d_x, summary, history);
+  });
+}

@@ -157,4 +157,31 @@ struct Workspace {
 };
 Workspace& workspace();
 
+// ------------------------------------------------------------- scratch arena
+//
+// While an ArenaScope is alive, Workspace::get and reduce_partials carve
+// their buffers out of one device region given to the scope instead of
+// allocating: each workspace slot gets a 256-byte-aligned piece on first
+// use, the reduction partials one piece sized up front.  A request that
+// does not fit throws SPK_INVALID_ARGUMENT; nothing is ever freed (the
+// region belongs to the caller).  Host-side bookkeeping is a fixed array.
+constexpr int kArenaSlots = 160;
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_round(size_t b) {
+  return (b + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+}
+class ArenaScope {
+ public:
+  // partials: room reserved for reduce_partials (doubles)
+  ArenaScope(void* base, size_t bytes, size_t partials);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+};
+// The library's static scratch array on the current device: its address
+// (looked up with hipGetSymbolAddress once per device, when the library
+// initialises that device) and its size.
+void* scratch_ptr();
+size_t scratch_bytes();
+
 }  // namespace spk

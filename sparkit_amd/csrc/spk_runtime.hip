@@ -8,7 +8,39 @@
 
 namespace spk {
 
+// Device scratch for the entry points that allocate nothing
+// (spk_cg_solve_noalloc).  In host code `scratch` names a host-side stand-in,
+// not device memory: every host use goes through the address
+// hipGetSymbolAddress returned for the current device (g_scratch_dev).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* slot_ptr[kArenaSlots] = {};
+  size_t slot_bytes[kArenaSlots] = {};
+  double* partials = nullptr;
+  size_t partials_count = 0;
+};
+Arena* g_arena = nullptr;
+
+void* arena_carve(Arena& a, size_t bytes) {
+  const size_t need = arena_round(bytes > 0 ? bytes : 1);
+  if (need > a.cap - a.used)
+    throw Error{SPK_INVALID_ARGUMENT,
+                "no-allocation scratch exhausted: " + std::to_string(a.used) +
+                    " + " + std::to_string(need) + " bytes > " +
+                    std::to_string(a.cap)};
+  void* p = a.base + a.used;
+  a.used += need;
+  return p;
+}
+
 thread_local std::string g_last_error;
 Context g_ctx;
 std::mutex g_init_mutex;
@@ -24,6 +56,12 @@ void init_device(int device) {
   SPK_REQUIRE(device >= 0 && device < count, SPK_INVALID_ARGUMENT,
               "device index out of range");
   SPK_HIP_TRY(hipSetDevice(device));
+  SPK_REQUIRE(device < kMaxDevices, SPK_INVALID_ARGUMENT, "device index too large");
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    SPK_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = p;
+  }
   hipDeviceProp_t prop;
   SPK_HIP_TRY(hipGetDeviceProperties(&prop, device));
   g_ctx.device = device;
@@ -51,7 +89,39 @@ hipStream_t stream() {
   return g_user_stream ? g_user_stream : c.stream;
 }
 
+void* scratch_ptr() {
+  Context& c = ctx();
+  void* p = g_scratch_dev[c.device];
+  SPK_REQUIRE(p, SPK_HIP_ERROR, "scratch address unknown for this device");
+  return p;
+}
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+ArenaScope::ArenaScope(void* base, size_t bytes, size_t partials) {
+  SPK_REQUIRE(!g_arena, SPK_INVALID_ARGUMENT, "nested no-allocation calls");
+  static Arena arena;  // fixed host bookkeeping, reset per scope
+  arena = Arena{};
+  arena.base = (char*)base;
+  arena.cap = bytes;
+  arena.partials = (double*)arena_carve(arena, partials * sizeof(double));
+  arena.partials_count = partials;
+  g_arena = &arena;
+}
+
+ArenaScope::~ArenaScope() { g_arena = nullptr; }
+
 void* Workspace::get(int slot, size_t bytes) {
+  if (g_arena) {
+    Arena& a = *g_arena;
+    SPK_REQUIRE(slot >= 0 && slot < kArenaSlots, SPK_INVALID_ARGUMENT,
+                "workspace slot out of range");
+    if (!a.slot_ptr[slot] || a.slot_bytes[slot] < bytes) {
+      a.slot_ptr[slot] = arena_carve(a, bytes);
+      a.slot_bytes[slot] = bytes;
+    }
+    return a.slot_ptr[slot];
+  }
   if ((size_t)slot >= bufs.size()) bufs.resize(slot + 1);
   bufs[slot].reserve(bytes);
   return bufs[slot].p;
@@ -64,6 +134,11 @@ Workspace& workspace() {
 
 double* reduce_partials(size_t count) {
   ctx();
+  if (g_arena) {
+    SPK_REQUIRE(count <= g_arena->partials_count, SPK_INVALID_ARGUMENT,
+                "no-allocation scratch: reduction partials exceed the reserve");
+    return g_arena->partials;
+  }
   if (count > g_partials->n) {
     // the old array may still be read by queued kernels
     SPK_HIP_TRY(hipStreamSynchronize(stream()));
