@@ -118,6 +118,11 @@ spk_status spk_csr_plan_info(spk_csr A, int64_t* nblocks,
 /* multiply(A, x) (sparse_blas.hpp:17-34): y = A x on device vectors of
  * `dtype` (SPK_F64, or SPK_F32 using the FP32 copy of the values). */
 spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y);
+/* y = A x and *d_result (a device double) = <x, y> summed in FP64 in a fixed
+ * order, in one SpMV with the sum in its epilogue: compute_step_size's
+ * apply-then-dot of conjugate_gradient.hpp:232-239 (square A). */
+spk_status spk_csr_spmv_dot(spk_csr A, int dtype, const void* d_x, void* d_y,
+                            double* d_result);
 
 /* Assembly on the device from `count` (row, col[, value]) entries held in
  * device memory (int64 indices, the reference's Index): the pattern of
@@ -257,6 +262,20 @@ spk_status spk_ilu0_get_jacobi_sweeps(spk_precond M, int* sweeps);
 spk_status spk_precond_create_identity(int64_t n, spk_precond* out);
 /* z = M r for an explicit sparse approximate inverse M (a store). */
 spk_status spk_precond_create_spmv(spk_csr M, spk_precond* out);
+/* A Krylov preconditioner for fgmres's flexible slot (fgmres.hpp:246-256;
+ * no reference counterpart): z = the iterate after up to `iterations`
+ * steps of conjugate_gradient (conjugate_gradient.hpp:122-312) on A z = r
+ * from z = 0 with M_inner (NULL: identity; e.g. a learned approximate
+ * inverse) in its preconditioner slot, stopping early once ||r - A z|| <
+ * tolerance ||r||.  `precision` is the inner solve's (SPK_F32: FP32
+ * vectors and SpMV, FP64 dot products and scalars); r and z are converted
+ * at the boundary.  Not a fixed operator: conjugate_gradient rejects it.
+ * A and M_inner must outlive it. */
+spk_status spk_precond_create_inner_cg(spk_csr A, spk_precond M_inner,
+                                       int64_t iterations, double tolerance,
+                                       int precision, spk_precond* out);
+/* PCG steps run by all applications of an inner-PCG preconditioner */
+spk_status spk_precond_inner_cg_iterations(spk_precond M, int64_t* total);
 spk_status spk_precond_destroy(spk_precond M);
 /* jacobi_apply (jacobi.hpp:42-50) / ilu_apply (ilu.hpp:173-179) / identity
  * copy / M r, on device vectors of `dtype`. */

@@ -38,7 +38,7 @@ __all__ = [
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "conjugate_gradient_noalloc", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
-    "true_relres", "dot", "axpby", "learned_solve", "LearnedConfig",
+    "true_relres", "dot", "axpby", "inner_cg", "InnerCg", "learned_solve", "LearnedConfig",
     "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
     "FALLBACK_ILU_GMRES", "FALLBACK_NONE",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
@@ -292,6 +292,13 @@ class CsrMatrix:
         return ch.value, [(int(tried[i]), float(us[i]))
                           for i in range(nt.value)]
 
+    def spmv_dot(self, x: DeviceVector, y: DeviceVector,
+                 result: DeviceVector):
+        """y = A x and result[0] = <x, y> (device), one fused launch plus
+        its finisher; nothing is synchronised."""
+        call("spk_csr_spmv_dot", self.h, _code(x.dtype), x.ptr, y.ptr,
+             result.ptr)
+
     def spmv(self, x: DeviceVector, y: DeviceVector):
         call("spk_csr_spmv", self.h, _code(x.dtype), x.ptr, y.ptr)
 
@@ -427,6 +434,28 @@ def milu0(A: CsrMatrix) -> Ilu0:
 def ilu_apply(F: Ilu0, r: DeviceVector, z: DeviceVector):
     """ilu_apply (ilu.hpp:173-179)."""
     F.apply(r, z)
+
+
+class InnerCg(Preconditioner):
+    """FGMRES's flexible-slot preconditioner: up to `iterations` PCG steps
+    on A z = r with M_inner, in `precision`."""
+
+    @property
+    def inner_iterations(self) -> int:
+        """PCG steps run by all applications so far."""
+        t = C.c_int64()
+        call("spk_precond_inner_cg_iterations", self.h, C.byref(t))
+        return t.value
+
+
+def inner_cg(A: CsrMatrix, M_inner=None, iterations: int = 200,
+             tolerance: float = 0.0, precision: int = F32) -> InnerCg:
+    """An inner-PCG preconditioner (spk_precond_create_inner_cg): a
+    nonlinear preconditioner, for fgmres."""
+    h = C.c_void_p()
+    call("spk_precond_create_inner_cg", A.h, _h(M_inner), int(iterations),
+         float(tolerance), int(precision), C.byref(h))
+    return InnerCg(h, "inner_cg", A.nrows, owner=(A, M_inner))
 
 
 def spai(M: CsrMatrix) -> Preconditioner:

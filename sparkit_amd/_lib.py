@@ -149,6 +149,7 @@ _SIGS = {
                                C.POINTER(vp)], None),
     "spk_csr_plan_info": ([vp, P_i64, P_i64], None),
     "spk_csr_spmv": ([vp, i32, vp, vp], None),
+    "spk_csr_spmv_dot": ([vp, i32, vp, vp, vp], None),
     "spk_csr_extract_diagonal": ([vp, vp], None),
     "spk_csr_assemble": ([i64, i64, i64, vp, vp, vp, C.POINTER(vp)], None),
     "spk_csr_permute": ([vp, vp, i32, C.POINTER(vp)], None),
@@ -156,6 +157,9 @@ _SIGS = {
     "spk_csr_get_panels": ([vp, P_i64], None),
     "spk_csr_tune_layout": ([vp, P_i64, vp, vp, P_i64], None),
     "spk_dot": ([i32, i64, vp, vp, P_dbl], None),
+    "spk_precond_create_inner_cg": ([vp, vp, i64, dbl, i32, C.POINTER(vp)],
+                                    None),
+    "spk_precond_inner_cg_iterations": ([vp, P_i64], None),
     "spk_learned_solve": ([vp, vp, vp, vp, vp, vp], None),
     "spk_axpby": ([i32, i64, dbl, vp, dbl, vp], None),
     "spk_convert": ([i64, i32, vp, i32, vp], None),

@@ -407,6 +407,25 @@ spk_status spk_csr_spmv(spk_csr A, int dtype, const void* d_x, void* d_y) {
   });
 }
 
+spk_status spk_csr_spmv_dot(spk_csr A, int dtype, const void* d_x, void* d_y,
+                            double* d_result) {
+  return guarded([&] {
+    SPK_REQUIRE(A && d_x && d_y && d_result, SPK_INVALID_ARGUMENT,
+                "null argument");
+    SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
+                "<x, A x> needs a square matrix");
+    SPK_REQUIRE(dtype == SPK_F32 || dtype == SPK_F64, SPK_INVALID_ARGUMENT,
+                "dtype must be SPK_F32 or SPK_F64");
+    hipStream_t s = stream();
+    if (dtype == SPK_F32)
+      launch_spmv(*A, (const float*)d_x,
+                  EpiDotX<float>{(const float*)d_x, (float*)d_y, d_result}, s);
+    else
+      launch_spmv(*A, (const double*)d_x,
+                  EpiDotX<double>{(const double*)d_x, (double*)d_y, d_result}, s);
+  });
+}
+
 spk_status spk_csr_set_panels(spk_csr A, int64_t panels) {
   return guarded([&] {
     SPK_REQUIRE(A && panels >= 1 && panels <= 4096, SPK_INVALID_ARGUMENT,

@@ -139,6 +139,21 @@ struct EpiStore {
   __device__ void finalize(double) const {}
 };
 
+// y = A x and <x, y> (square A): CG's q = A p with <p, q>
+template <typename T>
+struct EpiDotX {
+  static constexpr bool kReduce = true;
+  const T* __restrict__ x;
+  T* __restrict__ y;
+  double* out;
+  const int* done = nullptr;
+  __device__ double row(int i, T s) const {
+    y[i] = s;
+    return (double)x[i] * (double)s;
+  }
+  __device__ void finalize(double t) const { *out = t; }
+};
+
 // y = b - A x
 template <typename T>
 struct EpiResidual {

@@ -269,7 +269,11 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
       }, stop);
     }
     const int64_t m_eff = std::min(m, cfg->max_iterations - iters);
+    // a preconditioner that runs a solve of its own (inner PCG) costs far
+    // more than a state read: stop the cycle as soon as it has converged
+    const bool check_each = !right_id && right->kind == spk_precond_s::kInnerCg;
     for (int64_t j = 0; j < m_eff; ++j) {
+      if (check_each && j > 0 && st.download().stop) break;
       T* vj = V + (size_t)j * nn;
       T* vj1 = V + (size_t)(j + 1) * nn;
       T* zj = right_id ? vj : Z + (size_t)j * nn;

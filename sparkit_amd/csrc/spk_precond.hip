@@ -2,6 +2,7 @@
 #include "spk_precond.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 #include "spk_vec.hpp"
 
@@ -595,7 +596,58 @@ void spk_precond_s::apply(const T* in, T* out) {
     case kSpmv:
       launch_spmv(*M, in, EpiStore<T>{out}, stream());
       break;
+    case kInnerCg:
+      inner_cg_apply(in, out);
+      break;
   }
+}
+
+// z = up to inner_iters PCG steps on A z = r from z = 0 (the inner solve's
+// vectors in inner_precision; r and z converted at the boundary)
+template <typename U, typename T>
+static void inner_cg_run(spk_precond_s& P, const T* in, T* out,
+                         spk::DevBuf<U>& bb, spk::DevBuf<U>& zb) {
+  const int64_t n = P.n;
+  const U* b;
+  U* z;
+  if constexpr (std::is_same<U, T>::value) {
+    b = in;
+    z = out;
+    if (in == out) {  // the solve reads b while it writes z
+      bb.reserve((size_t)(n > 0 ? n : 1));
+      SPK_HIP_TRY(hipMemcpyAsync(bb.p, in, (size_t)n * sizeof(U),
+                                 hipMemcpyDeviceToDevice, stream()));
+      b = bb.p;
+    }
+  } else {
+    bb.reserve((size_t)(n > 0 ? n : 1));
+    zb.reserve((size_t)(n > 0 ? n : 1));
+    U* bp = bb.p;
+    launch_map(n, [=] __device__(int64_t i) { bp[i] = (U)in[i]; });
+    b = bp;
+    z = zb.p;
+  }
+  launch_map(n, [=] __device__(int64_t i) { z[i] = U(0); });
+  spk_solver_config c{P.inner_tol, P.inner_iters, P.inner_iters, 0,
+                      P.inner_precision, 0, 0};
+  spk_summary sum{};
+  const spk_status st = spk_cg_solve(P.inner_A, P.inner_M, nullptr, &c, b, z,
+                                     &sum, nullptr);
+  if (st != SPK_OK && st != SPK_BREAKDOWN)
+    throw Error{st, std::string("inner PCG: ") + spk_last_error()};
+  P.inner_total += sum.computed_iterations;
+  if constexpr (!std::is_same<U, T>::value) {
+    const U* zp = z;
+    launch_map(n, [=] __device__(int64_t i) { out[i] = (T)zp[i]; });
+  }
+}
+
+template <typename T>
+void spk_precond_s::inner_cg_apply(const T* in, T* out) {
+  if (inner_precision == SPK_F32)
+    inner_cg_run<float, T>(*this, in, out, ib32, iz32);
+  else
+    inner_cg_run<double, T>(*this, in, out, ib64, iz64);
 }
 
 template void spk_precond_s::apply<double>(const double*, double*);
@@ -704,6 +756,41 @@ spk_status spk_precond_create_spmv(spk_csr Mst, spk_precond* out) {
     M->n = Mst->nrows;
     M->M = Mst;
     *out = M;
+  });
+}
+
+spk_status spk_precond_create_inner_cg(spk_csr A, spk_precond M_inner,
+                                       int64_t iterations, double tolerance,
+                                       int precision, spk_precond* out) {
+  return guarded([&] {
+    SPK_REQUIRE(A && out, SPK_INVALID_ARGUMENT, "null argument");
+    SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
+                "inner PCG requires a square matrix");
+    SPK_REQUIRE(!M_inner || M_inner->n == A->nrows, SPK_INVALID_ARGUMENT,
+                "inner preconditioner size mismatch");
+    SPK_REQUIRE(!M_inner || M_inner->kind != spk_precond_s::kInnerCg,
+                SPK_INVALID_ARGUMENT, "inner PCG cannot nest another one");
+    SPK_REQUIRE(iterations >= 1 && tolerance >= 0.0, SPK_INVALID_ARGUMENT,
+                "iterations >= 1, tolerance >= 0");
+    SPK_REQUIRE(precision == SPK_F32 || precision == SPK_F64,
+                SPK_INVALID_ARGUMENT, "precision must be SPK_F32 or SPK_F64");
+    auto* P = new spk_precond_s();
+    P->kind = spk_precond_s::kInnerCg;
+    P->n = A->nrows;
+    P->inner_A = A;
+    P->inner_M = M_inner;
+    P->inner_iters = iterations;
+    P->inner_tol = tolerance;
+    P->inner_precision = precision;
+    *out = P;
+  });
+}
+
+spk_status spk_precond_inner_cg_iterations(spk_precond M, int64_t* total) {
+  return guarded([&] {
+    SPK_REQUIRE(M && total && M->kind == spk_precond_s::kInnerCg,
+                SPK_INVALID_ARGUMENT, "not an inner PCG preconditioner");
+    *total = M->inner_total;
   });
 }
 

@@ -17,7 +17,7 @@
 #include "spk_csr.hpp"
 
 struct spk_precond_s {
-  enum Kind { kIdentity, kJacobi, kIlu, kSpmv } kind = kIdentity;
+  enum Kind { kIdentity, kJacobi, kIlu, kSpmv, kInnerCg } kind = kIdentity;
   int64_t n = 0;
   // Jacobi
   spk::DevBuf<double> inv_d64;
@@ -56,6 +56,18 @@ struct spk_precond_s {
   bool du_has32 = false;
   // SPAI
   spk_csr_s* M = nullptr;  // not owned
+  // Inner PCG (a Krylov preconditioner for FGMRES's flexible slot): z = the
+  // iterate of up to inner_iters PCG steps on inner_A z = r from z = 0 with
+  // inner_M in its preconditioner slot, in inner_precision, stopping at a
+  // relative residual of inner_tol.  inner_A and inner_M are not owned.
+  spk_csr_s* inner_A = nullptr;
+  spk_precond_s* inner_M = nullptr;
+  int64_t inner_iters = 0;
+  double inner_tol = 0.0;
+  int inner_precision = 1;  // SPK_F32
+  int64_t inner_total = 0;  // PCG steps run by all applications so far
+  spk::DevBuf<float> ib32, iz32;
+  spk::DevBuf<double> ib64, iz64;
 
   ~spk_precond_s();
   const float* inv_diag32();
@@ -80,6 +92,8 @@ struct spk_precond_s {
   void build_strict();
   template <typename T>
   void jacobi_apply(const T* in, T* out);
+  template <typename T>
+  void inner_cg_apply(const T* in, T* out);
 };
 
 template <>

@@ -288,3 +288,26 @@ def test_panels_with_few_and_no_rows(O, spk):
     y32 = spk.DeviceVector.zeros(n, np.float32)
     Ad.spmv(spk.DeviceVector.from_numpy(x32), y32)
     assert np.array_equal(y32.numpy(), O.spmv(A, x32))
+
+
+def test_spmv_dot_matches_oracle(O, spk):
+    """spk_csr_spmv_dot: y = A x bit for bit as multiply (sparse_blas.hpp:
+    17-34) and <x, y> (compute_step_size, conjugate_gradient.hpp:232-239)
+    within FP64 rounding of the oracle's left-to-right sum; with column
+    panels too; a non-square store is rejected."""
+    R = O.poisson_3d(30, 28, 26)
+    A = spk.CsrMatrix.from_oracle(R)
+    x = O.uniform_pm1(3, R.nrows)
+    y_ref = O.spmv(R, x)
+    d_ref = float(np.dot(x, y_ref))
+    for panels in (1, 3):
+        A.set_panels(panels)
+        y = spk.DeviceVector.zeros(R.nrows)
+        r = spk.DeviceVector.zeros(1)
+        A.spmv_dot(spk.DeviceVector.from_numpy(x), y, r)
+        assert np.array_equal(y.numpy(), y_ref)
+        assert abs(r.numpy()[0] - d_ref) <= 1e-12 * np.abs(x * y_ref).sum()
+    B = spk.CsrMatrix.from_host(3, 4, [0, 1, 2, 3], [0, 1, 3], [1.0, 2.0, 3.0])
+    with pytest.raises(spk.InvalidArgument):
+        B.spmv_dot(spk.DeviceVector.zeros(4), spk.DeviceVector.zeros(3),
+                   spk.DeviceVector.zeros(1))
