@@ -100,12 +100,18 @@ def collect_cpu_legs(state, deadline_s):
     still running then is stopped and reported unfinished."""
     out = {}
     for group, p in state["procs"]:
-        left = max(1.0, deadline_s - (time.perf_counter() - state["t0"]))
-        try:
-            so, se = p.communicate(timeout=left)
-        except subprocess.TimeoutExpired:
-            p.kill()
-            so, se = p.communicate()
+        so = se = None
+        while so is None:  # a progress line every 30 s while a leg runs
+            left = deadline_s - (time.perf_counter() - state["t0"])
+            try:
+                so, se = p.communicate(timeout=max(1.0, min(30.0, left)))
+            except subprocess.TimeoutExpired:
+                if left <= 30.0:
+                    p.kill()
+                    so, se = p.communicate()
+                else:
+                    log(f"cpu baseline: legs {','.join(group)} still running "
+                        f"({time.perf_counter() - state['t0']:.0f} s)")
         for line in so.splitlines():
             try:
                 r = json.loads(line)
@@ -487,11 +493,11 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     lcfg = spk.LearnedConfig(tolerance=1e-8, restart=30, max_iterations=50000)
     guard = {}
 
-    def guarded(name, M):
+    def guarded(name, M, warm):
         def fn(x):
             guard[name] = spk.learned_solve(b, x, lcfg, A, M)
             return guard[name]
-        run(name, fn)
+        run(name, fn, warm)
         g = guard[name]
         res[name].update(method=g.method, reasons=g.reasons,
                          learned_iterations=g.learned_iterations,
@@ -511,12 +517,12 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
             inner[name] = P
             res[name]["inner_iterations_per_solve"] = \
                 P.inner_iterations // 2  # warm-up + timed solve
-    guarded("guarded_fgmres30_learned", Ml)
+    guarded("guarded_fgmres30_learned", Ml, False)  # FGMRES above: warm
     poisoned = spk.SpaiModel(1)
     pp = model.params
     pp[SPAI_W3:SPAI_B3] = np.nan
     poisoned.params = pp
-    guarded("guarded_fgmres30_nan_network", poisoned.preconditioner(A))
+    guarded("guarded_fgmres30_nan_network", poisoned.preconditioner(A), True)
     # the metric's own ratio: learned-SPAI FGMRES against Jacobi-PCG, both
     # to the 1e-8 FP64 true residual
     res["plain_learned_fgmres_vs_jacobi_pcg"] = speedup(

@@ -184,9 +184,6 @@ struct EpiFinalize {  // adapts Epi::finalize to the finisher's fin(totals)
 
 // ------------------------------------------------------------- kernel
 
-// the word a launch without a done flag tests (never set)
-static __device__ int kNeverDone = 0;
-
 template <typename T>
 struct Pair;
 template <>
@@ -208,17 +205,10 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     const T* init) {
   __shared__ T prod[kNnzBlock + 2];
   __shared__ double red[kSpmvBlock / 64];
+  if (epi.done && *epi.done) return;
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
-  // The done flag and the block's plan entries are independent loads, and
-  // both are issued before one wait: a separate flag test first would put a
-  // memory round trip in front of every block's streams.  (A null flag
-  // reads a zero word; the exit test also reads the plan, so its loads stay
-  // ahead of the branch.)
-  const int* stop_flag = epi.done ? epi.done : &kNeverDone;
   const int2 p0 = A.plan[b], p1 = A.plan[b + 1];
-  const int stop = *stop_flag;
-  if (stop != 0 && p1.x >= p0.x) return;
   const int r0 = p0.x, base = p0.y, nrows = p1.x - p0.x, cnt = p1.y - p0.y;
   double acc = 0.0;
   if (cnt > kNnzBlock) {
@@ -292,7 +282,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     }
   }
   if constexpr (Epi::kReduce) {
-    double t = block_reduce_sum<double, kSpmvBlock>(acc, red);
+    const double t = block_sum_to_thread0<kSpmvBlock>(acc, red);
     if (tid == 0) partials[b] = t;
   }
 }

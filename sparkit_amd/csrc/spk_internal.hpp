@@ -133,6 +133,47 @@ __device__ __forceinline__ T block_reduce_sum(T v, T* smem /* BLOCK/64 */) {
   return s;
 }
 
+// Wavefront sum with DPP lane moves (quad swaps, half-row and row mirrors,
+// row broadcasts) instead of LDS permutes: a fixed order, the total in lane
+// 63 (other lanes hold partial sums).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xF>(v);  // row_mirror: every lane holds its row's sum
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over a block of BLOCK threads, valid in thread 0 only (fixed order).
+template <int BLOCK>
+__device__ __forceinline__ double block_sum_to_thread0(double v, double* smem) {
+  constexpr int W = BLOCK / 64;
+  v = wave_sum_dpp(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    s = smem[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) s += smem[i];
+  }
+  return s;
+}
+
 // Coherent load of a value another block may have written during the same
 // kernel (different XCDs do not share an L2).
 __device__ __forceinline__ double load_partial(const double* p) {

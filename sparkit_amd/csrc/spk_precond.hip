@@ -148,6 +148,7 @@ __device__ __forceinline__ T sentinel_value() {
 }
 
 constexpr unsigned kSpinLimit = 1u << 22;
+constexpr int kSweepBatch = 4;  // dependencies a sweep row loads together
 constexpr int kErrorWords = 64;  // sweep status word + guard words
 // sweep schedule thresholds (see spk_precond_s::level_launch)
 constexpr int64_t kLevelLaunchRows = 65536;
@@ -181,27 +182,48 @@ __global__ __launch_bounds__(256) void sweep_kernel(
     T s = b[i];
     if (reset_b) b[i] = sentinel_value<T>();
     if (mark) mark[i] = sentinel_value<T>();
+    // The dependencies are polled kSweepBatch at a time: all of a batch's
+    // loads are issued together (one round trip instead of one per
+    // dependency), and the batch is consumed in storage order once every
+    // value in it is ready -- the reference's arithmetic, unchanged.
     int p = LOWER ? p0 : p0 + 1;
     unsigned spins = 0;
-    bool done = false;
-    while (!done) {
-      if (p < q1) {
-        T xv;
-        if (load_ready(x + ci[p], &xv)) {
-          s = s - v[p] * xv;
-          ++p;
-        } else if (++spins > kSpinLimit) {
-          atomicOr(error, LOWER ? 1 : 2);
-          p = q1;
-        } else {
+    while (p < q1) {
+      const int cnt = q1 - p < kSweepBatch ? q1 - p : kSweepBatch;
+      int c[kSweepBatch];
+      T xv[kSweepBatch];
+      bool ok[kSweepBatch];
+#pragma unroll
+      for (int k = 0; k < kSweepBatch; ++k) {
+        ok[k] = k >= cnt;
+        c[k] = k < cnt ? ci[p + k] : 0;
+      }
+      bool all = false;
+      while (!all) {
+        all = true;
+#pragma unroll
+        for (int k = 0; k < kSweepBatch; ++k) {
+          if (!ok[k]) ok[k] = load_ready(x + c[k], &xv[k]);
+          all = all && ok[k];
+        }
+        if (!all) {
+          if (++spins > kSpinLimit) {
+            atomicOr(error, LOWER ? 1 : 2);
+            break;
+          }
           __builtin_amdgcn_s_sleep(2);  // back off: fewer polls in flight
         }
       }
-      if (p >= q1) {
-        store_ready(x + i, s / v[pd]);
-        done = true;
+      if (!all) {  // expired wait: publish what there is (reported)
+        p = q1;
+        break;
       }
+#pragma unroll
+      for (int k = 0; k < kSweepBatch; ++k)
+        if (k < cnt) s = s - v[p + k] * xv[k];
+      p += cnt;
     }
+    store_ready(x + i, s / v[pd]);
     // keep this row's publication ahead of the next row's waits
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
   }
