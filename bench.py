@@ -198,21 +198,31 @@ def cpu_baseline(legs, state, inputs):
 POWERLAW_ITERS_CPU = 11  # the power-law solve's Jacobi-PCG count
 
 
+def layout_name(code):
+    """'plain', 'panels:P' or 'tiles:SxW' for a layout code."""
+    if code == 1:
+        return "plain"
+    if code > 1:
+        return f"panels:{code}"
+    return f"tiles:{-code // 65536}x{-code % 65536}"
+
+
 class Input:
     def __init__(self, spk, name, A, tune=True, panels=None):
         self.name, self.A = name, A
         self.n, self.nnz = A.nrows, A.nnz
         self.layout = None
         if panels is not None:  # a fixed layout (profiling runs)
-            A.set_panels(panels)
+            A.set_layout(panels)
             self.layout = {"panels": panels, "fixed": True}
         elif tune:  # SpMV layout (column panels) chosen by timing, setup only
             t0 = time.perf_counter()
             chosen, tried = A.tune_layout()
-            self.layout = {"panels": chosen,
-                           "tried_us": {str(p): round(us, 1) for p, us in tried},
+            self.layout = {"panels": chosen, "name": layout_name(chosen),
+                           "tried_us": {layout_name(p): round(us, 1)
+                                        for p, us in tried},
                            "tune_s": round(time.perf_counter() - t0, 2)}
-            log(f"{name}: SpMV layout {chosen} panel(s) from {tried}")
+            log(f"{name}: SpMV layout {layout_name(chosen)} from {tried}")
         self.b = spk.uniform_pm1(42, self.n)
         self.x = spk.DeviceVector.zeros(self.n)
         self.J = spk.jacobi(A)
@@ -852,7 +862,9 @@ def main():
                     help="keep the plain CSR layout (no column panels)")
     ap.add_argument("--panels", default=None,
                     help="fixed SpMV layouts instead of the tuner's choice, "
-                         "one per input: e.g. 1,16 (profiling runs)")
+                         "one per input: e.g. 1,16 (profiling runs); a "
+                         "negative code -(slots * 65536 + window) selects "
+                         "window tiles")
     ap.add_argument("--sections",
                     default="p2d,mp,spai,batched,krylov,mmio,rcm",
                     help="extra measurements after the timed steps: "

@@ -200,10 +200,26 @@ spk_status spk_csr_permute(spk_csr A, const int64_t* d_perm, int mode,
  * values are copied when the panels are made: call again after changing
  * them.  No reference counterpart (Compressed_row_matrix has one layout). */
 spk_status spk_csr_set_panels(spk_csr A, int64_t panels);
+/* the layout code: panels (>= 1), or -(slots * 65536 + window) for window
+ * tiles */
 spk_status spk_csr_get_panels(spk_csr A, int64_t* panels);
+/* Window tiles (no reference counterpart), for matrices whose rows reach
+ * across all columns: rows in groups of 1024 * slots (slots = 8 or 16 rows
+ * per thread), and per window of `window` columns (256..8192, a multiple
+ * of 64) each thread's entries of its rows stored together; the SpMV stages
+ * each x window in LDS, so every gather is an LDS read.  A row's sum runs
+ * in column order with separately rounded products and sums: the
+ * reference's storage-order sum (sparse_blas.hpp:26-31) when columns
+ * ascend within rows (required: SPK_INVALID_ARGUMENT otherwise).  Replaces
+ * column panels; the values are copied. */
+spk_status spk_csr_set_window_tiles(spk_csr A, int64_t slots, int64_t window);
+/* a layout by its code (as spk_csr_get_panels reports it) */
+spk_status spk_csr_set_layout(spk_csr A, int64_t code);
 /* times one FP64 SpMV for 1, 4, 8, 12, 16, 24, 32 and 64 panels (those
- * leaving at least 32768 columns per panel), keeps the fastest; tried /
- * tried_us (may be NULL, room for 8) list the candidates */
+ * leaving at least 32768 columns per panel) and, for matrices of at least
+ * 131072 columns and 32 entries per row on average, three window-tile
+ * shapes; keeps the fastest.  tried / tried_us (may be NULL, room for 11)
+ * list the candidates' layout codes and times */
 spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
                                float* tried_us, int64_t* ntried);
 

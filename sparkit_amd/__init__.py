@@ -272,10 +272,19 @@ class CsrMatrix:
 
     @property
     def panels(self) -> int:
-        """Number of column panels the SpMV runs over (1: plain store)."""
+        """The SpMV layout code: the number of column panels (1: plain
+        store), or -(slots * 65536 + window) for window tiles."""
         p = C.c_int64()
         call("spk_csr_get_panels", self.h, C.byref(p))
         return p.value
+
+    def set_window_tiles(self, slots: int = 16, window: int = 8192):
+        """Window tiles: x windows staged in LDS (spk_csr_set_window_tiles)."""
+        call("spk_csr_set_window_tiles", self.h, int(slots), int(window))
+
+    def set_layout(self, code: int):
+        """A layout by its code (panels, or a negative window-tile code)."""
+        call("spk_csr_set_layout", self.h, int(code))
 
     def set_panels(self, panels: int):
         call("spk_csr_set_panels", self.h, int(panels))
@@ -284,8 +293,8 @@ class CsrMatrix:
         """Times the SpMV over 1..64 column panels and keeps the fastest;
         returns (chosen, [(panels, us), ...])."""
         ch, nt = C.c_int64(), C.c_int64()
-        tried = np.zeros(8, np.int64)
-        us = np.zeros(8, np.float32)
+        tried = np.zeros(16, np.int64)
+        us = np.zeros(16, np.float32)
         call("spk_csr_tune_layout", self.h, C.byref(ch),
              tried.ctypes.data_as(C.c_void_p), us.ctypes.data_as(C.c_void_p),
              C.byref(nt))

@@ -156,7 +156,179 @@ __global__ void panel_copy_kernel(int64_t n, const int32_t* __restrict__ start,
   }
 }
 
+// Window tiles: one thread per (group, owner thread) walks its S rows'
+// cursors window by window.  COUNT: writes each (window, thread) segment's
+// length into off and flags rows whose columns do not ascend; FILL: writes
+// the entries at base[g] + off[window * T + t] (off scanned by then).
+template <int S, bool FILL>
+__global__ void xt_walk_kernel(int64_t nrows, const int32_t* __restrict__ rp,
+                               const int32_t* __restrict__ ci,
+                               const double* __restrict__ v, int64_t groups,
+                               int64_t windows, int window,
+                               int32_t* __restrict__ off,
+                               const int64_t* __restrict__ base,
+                               uint32_t* __restrict__ idx, double* __restrict__ xv,
+                               int* __restrict__ bad) {
+  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gt >= groups * kXtThreads) return;
+  const int64_t g = gt / kXtThreads;
+  const int t = (int)(gt % kXtThreads);
+  int cur[S], end[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int64_t r = g * (int64_t)(kXtThreads * S) + t + (int64_t)s * kXtThreads;
+    cur[s] = r < nrows ? rp[r] : 0;
+    end[s] = r < nrows ? rp[r + 1] : 0;
+  }
+  int32_t* o = off + g * (windows * kXtThreads + 1);
+  for (int64_t w = 0; w < windows; ++w) {
+    const int64_t c0 = w * (int64_t)window, limit = c0 + window;
+    int64_t p = FILL ? base[g] + o[w * kXtThreads + t] : 0;
+    int count = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      while (cur[s] < end[s] && (int64_t)ci[cur[s]] < limit) {
+        const int k = cur[s];
+        if (FILL) {
+          idx[p] = ((uint32_t)s << 16) | (uint32_t)(ci[k] - c0);
+          xv[p] = v[k];
+          ++p;
+        } else {
+          ++count;
+        }
+        ++cur[s];
+      }
+    }
+    if (!FILL) o[w * kXtThreads + t] = count;
+  }
+  if (!FILL) {
+    // ascending columns within each owned row (the walk above relies on it)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int64_t r = g * (int64_t)(kXtThreads * S) + t + (int64_t)s * kXtThreads;
+      if (r >= nrows) continue;
+      for (int k = rp[r] + 1; k < rp[r + 1]; ++k)
+        if (ci[k] <= ci[k - 1]) atomicOr(bad, 1);
+      if (cur[s] != end[s]) atomicOr(bad, 2);  // a column beyond the last window
+    }
+  }
+}
+
+// In place: each group's segment lengths -> exclusive offsets, the group's
+// total in the last slot.  One workgroup per group.
+__global__ __launch_bounds__(1024) void xt_scan_kernel(int32_t* __restrict__ off,
+                                                       int64_t N) {
+  __shared__ int64_t sums[1024];
+  int32_t* a = off + blockIdx.x * (N + 1);
+  const int64_t chunk = (N + 1023) / 1024;
+  const int64_t lo = threadIdx.x * chunk;
+  const int64_t hi = lo + chunk < N ? lo + chunk : N;
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += a[i];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int64_t t = sums[i];
+      sums[i] = run;
+      run += t;
+    }
+    a[N] = (int32_t)run;
+  }
+  __syncthreads();
+  int64_t run = sums[threadIdx.x];
+  for (int64_t i = lo; i < hi; ++i) {
+    const int32_t t = a[i];
+    a[i] = (int32_t)run;
+    run += t;
+  }
+}
+
+__global__ void xt_totals_kernel(const int32_t* __restrict__ off, int64_t N,
+                                 int64_t groups, int64_t* __restrict__ tot) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < groups) tot[g] = off[g * (N + 1) + N];
+}
+
 }  // namespace
+
+void build_tiles(spk_csr_s& A, int S, int window) {
+  SPK_REQUIRE(S == 8 || S == 16, SPK_INVALID_ARGUMENT,
+              "window tiles: 8 or 16 rows per thread");
+  SPK_REQUIRE(window >= 256 && window <= kXtMaxWindow && window % 64 == 0,
+              SPK_INVALID_ARGUMENT, "window tiles: window in [256, 8192], a multiple of 64");
+  A.drop_panels();
+  if (A.nrows == 0 || A.nnz == 0) return;
+  hipStream_t s = stream();
+  const int64_t rows_per_group = (int64_t)kXtThreads * S;
+  const int64_t groups = (A.nrows + rows_per_group - 1) / rows_per_group;
+  const int64_t windows = (A.ncols + window - 1) / window;
+  const int64_t N = windows * kXtThreads;
+  SPK_REQUIRE(N < ((int64_t)1 << 31) / 2, SPK_INVALID_ARGUMENT,
+              "window tiles: too many windows");
+  DevBuf<int32_t> off((size_t)(groups * (N + 1)));
+  DevBuf<int> bad(1);
+  DevBuf<int64_t> tot((size_t)groups);
+  SPK_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  const int64_t threads = groups * kXtThreads;
+  const unsigned grid = (unsigned)((threads + 255) / 256);
+  auto walk = [&](bool fill, const int64_t* base, uint32_t* idx, double* xv) {
+    if (S == 16) {
+      if (fill)
+        hipLaunchKernelGGL((xt_walk_kernel<16, true>), dim3(grid), dim3(256), 0, s,
+                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                           (const double*)A.v64.p, groups, windows, window, off.p,
+                           base, idx, xv, bad.p);
+      else
+        hipLaunchKernelGGL((xt_walk_kernel<16, false>), dim3(grid), dim3(256), 0, s,
+                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                           (const double*)A.v64.p, groups, windows, window, off.p,
+                           base, idx, xv, bad.p);
+    } else {
+      if (fill)
+        hipLaunchKernelGGL((xt_walk_kernel<8, true>), dim3(grid), dim3(256), 0, s,
+                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                           (const double*)A.v64.p, groups, windows, window, off.p,
+                           base, idx, xv, bad.p);
+      else
+        hipLaunchKernelGGL((xt_walk_kernel<8, false>), dim3(grid), dim3(256), 0, s,
+                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                           (const double*)A.v64.p, groups, windows, window, off.p,
+                           base, idx, xv, bad.p);
+    }
+    SPK_HIP_TRY(hipGetLastError());
+  };
+  walk(false, nullptr, nullptr, nullptr);
+  int hbad = 0;
+  SPK_HIP_TRY(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPK_HIP_TRY(hipStreamSynchronize(s));
+  SPK_REQUIRE(hbad == 0, SPK_INVALID_ARGUMENT,
+              "window tiles need ascending columns within every row");
+  hipLaunchKernelGGL(xt_scan_kernel, dim3((unsigned)groups), dim3(1024), 0, s, off.p, N);
+  hipLaunchKernelGGL(xt_totals_kernel, dim3((unsigned)((groups + 255) / 256)),
+                     dim3(256), 0, s, (const int32_t*)off.p, N, groups, tot.p);
+  SPK_HIP_TRY(hipGetLastError());
+  std::vector<int64_t> htot((size_t)groups), hbase((size_t)groups + 1);
+  SPK_HIP_TRY(hipMemcpyAsync(htot.data(), tot.p, groups * 8, hipMemcpyDeviceToHost, s));
+  SPK_HIP_TRY(hipStreamSynchronize(s));
+  hbase[0] = 0;
+  for (int64_t g = 0; g < groups; ++g) hbase[(size_t)g + 1] = hbase[(size_t)g] + htot[(size_t)g];
+  SPK_REQUIRE(hbase[(size_t)groups] == A.nnz, SPK_HIP_ERROR,
+              "window tiles: entry count mismatch");
+  A.xt_base.alloc((size_t)groups + 1);
+  SPK_HIP_TRY(hipMemcpyAsync(A.xt_base.p, hbase.data(), (groups + 1) * 8,
+                             hipMemcpyHostToDevice, s));
+  A.xt_idx.alloc((size_t)A.nnz);
+  A.xt_v64.alloc((size_t)A.nnz);
+  walk(true, A.xt_base.p, A.xt_idx.p, A.xt_v64.p);
+  SPK_HIP_TRY(hipStreamSynchronize(s));
+  A.xt_off = std::move(off);
+  A.xt_slots = S;
+  A.xt_window = window;
+  A.xt_groups = groups;
+  A.xt_windows = windows;
+}
 
 void build_panels(spk_csr_s& A, int P) {
   A.drop_panels();
@@ -244,6 +416,18 @@ void spk_csr_s::allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_) {
   hipStream_t s = spk::stream();
   SPK_HIP_TRY(hipMemsetAsync(ci.p + nnz, 0, spk::kPad * sizeof(int32_t), s));
   SPK_HIP_TRY(hipMemsetAsync(v64.p + nnz, 0, spk::kPad * sizeof(double), s));
+}
+
+const float* spk_csr_s::tiles32() {
+  if (!xt_has32) {
+    xt_v32.alloc((size_t)(nnz > 0 ? nnz : 1));
+    if (nnz > 0)
+      hipLaunchKernelGGL(spk::f64_to_f32_kernel, dim3(spk::grid_for(nnz)),
+                         dim3(256), 0, spk::stream(), xt_v64.p, xt_v32.p, nnz);
+    SPK_HIP_TRY(hipGetLastError());
+    xt_has32 = true;
+  }
+  return xt_v32.p;
 }
 
 const float* spk_csr_s::vals32() {
@@ -438,7 +622,33 @@ spk_status spk_csr_set_panels(spk_csr A, int64_t panels) {
 spk_status spk_csr_get_panels(spk_csr A, int64_t* panels) {
   return guarded([&] {
     SPK_REQUIRE(A && panels, SPK_INVALID_ARGUMENT, "null argument");
-    *panels = A->panels.empty() ? 1 : (int64_t)A->panels.size();
+    if (A->xt_groups)
+      *panels = -((int64_t)A->xt_slots * 65536 + A->xt_window);
+    else
+      *panels = A->panels.empty() ? 1 : (int64_t)A->panels.size();
+  });
+}
+
+spk_status spk_csr_set_window_tiles(spk_csr A, int64_t slots, int64_t window) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    build_tiles(*A, (int)slots, (int)window);
+  });
+}
+
+// layout codes of spk_csr_tune_layout / spk_csr_set_layout: P > 0 column
+// panels, -(slots * 65536 + window) window tiles
+static void apply_layout(spk_csr_s& A, int64_t code) {
+  if (code > 0) build_panels(A, (int)code);
+  else build_tiles(A, (int)(-code / 65536), (int)(-code % 65536));
+}
+
+spk_status spk_csr_set_layout(spk_csr A, int64_t code) {
+  return guarded([&] {
+    SPK_REQUIRE(A && code != 0, SPK_INVALID_ARGUMENT, "bad argument");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    apply_layout(*A, code);
   });
 }
 
@@ -450,18 +660,28 @@ spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
     DevBuf<double> x((size_t)std::max<int64_t>(A->ncols, 1));
     DevBuf<double> y((size_t)std::max<int64_t>(A->nrows, 1));
     SPK_HIP_TRY(hipMemsetAsync(x.p, 0, x.n * 8, s));
-    std::vector<int> cands{1};
+    std::vector<int64_t> cands{1};
     for (int P : {4, 8, 12, 16, 24, 32, 64})
       if (A->ncols >= (int64_t)P * 32768) cands.push_back(P);
+    // window tiles stage every x window of a row group in LDS, so one call
+    // reads groups * ncols x values where a gather reads about nnz: they are
+    // offered only where the windows are at least that dense (a power-law
+    // matrix of 4.2M rows and 38 entries a row reads 6x more x through them
+    // and runs 3x slower than 16 column panels)
+    const int64_t groups16 = (A->nrows + 16 * kXtThreads - 1) / (16 * kXtThreads);
+    if (A->ncols >= 16 * 8192 && A->nrows >= 2 * kXtThreads * 16 &&
+        groups16 * A->ncols <= A->nnz)
+      for (int64_t code : {16 * 65536 + 8192, 8 * 65536 + 8192, 16 * 65536 + 4096})
+        cands.push_back(-code);
     hipEvent_t e0, e1;
     SPK_HIP_TRY(hipEventCreate(&e0));
     SPK_HIP_TRY(hipEventCreate(&e1));
-    int best = 1;
+    int64_t best = 1;
     float best_us = 1e30f;
     int64_t k = 0;
     try {
-      for (int P : cands) {
-        build_panels(*A, P);
+      for (int64_t P : cands) {
+        apply_layout(*A, P);
         for (int w = 0; w < 2; ++w) launch_spmv(*A, (const double*)x.p, EpiStore<double>{y.p}, s);
         SPK_HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < 5; ++r)
@@ -486,7 +706,7 @@ spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    build_panels(*A, best);
+    apply_layout(*A, best);
     if (ntried) *ntried = k;
     *chosen = best;
   });

@@ -28,6 +28,8 @@
 // of a 7-point stencil is then shared through the Infinity Cache).
 #pragma once
 
+#include <type_traits>
+
 #include "spk_internal.hpp"
 #include "spk_vec.hpp"
 
@@ -71,6 +73,22 @@ struct spk_csr_s {
   spk::DevBuf<int32_t> rows;  // compressed store: matrix row of each row
   spk::DevBuf<double> pacc64;
   spk::DevBuf<float> pacc32;
+  // Window tiles (xt_groups == 0: none), for matrices whose rows scatter
+  // over all columns.  Rows are cut into groups of kXtThreads * xt_slots;
+  // thread t of a group owns its rows t + s * kXtThreads (slot s).  For
+  // every window of xt_window columns each thread's entries of its rows in
+  // that window are stored together, in (slot, column) order, as
+  // (slot << 16 | column - window start, value); the SpMV stages the x
+  // window in LDS, so every gather is an LDS read, and a row's sum runs in
+  // column order (the CSR storage order when rows are sorted).
+  int xt_slots = 0, xt_window = 0;
+  int64_t xt_groups = 0, xt_windows = 0;
+  spk::DevBuf<int64_t> xt_base;  // first entry of each group (groups + 1)
+  spk::DevBuf<int32_t> xt_off;   // per group, (windows * threads + 1) offsets
+  spk::DevBuf<uint32_t> xt_idx;
+  spk::DevBuf<double> xt_v64;
+  spk::DevBuf<float> xt_v32;
+  bool xt_has32 = false;
 
   spk_csr_s() = default;
   spk_csr_s(const spk_csr_s&) = delete;
@@ -79,7 +97,19 @@ struct spk_csr_s {
   void drop_panels() {
     for (auto* q : panels) delete q;
     panels.clear();
+    drop_tiles();
   }
+  void drop_tiles() {
+    xt_slots = xt_window = 0;
+    xt_groups = xt_windows = 0;
+    xt_base.release();
+    xt_off.release();
+    xt_idx.release();
+    xt_v64.release();
+    xt_v32.release();
+    xt_has32 = false;
+  }
+  const float* tiles32();
   // allocates rp, ci and v64 (with the zeroed kPad tail)
   void allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_);
   const float* vals32();
@@ -121,6 +151,10 @@ void upload_plan(spk_csr_s& A, const std::vector<int32_t>& blocks);
 void finish_store(spk_csr_s& A, bool validate_cols);
 // Split A into `panels` column panels (1: none); values are copied.
 void build_panels(spk_csr_s& A, int panels);
+// Window tiles with `slots` rows per thread (8 or 16) and `window` columns
+// per x window (<= kXtMaxWindow); drops panels.  SPK_INVALID_ARGUMENT if a
+// row's columns are not ascending.
+void build_tiles(spk_csr_s& A, int slots, int window);
 
 // ------------------------------------------------------------- epilogues
 //
@@ -287,17 +321,106 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
   }
 }
 
+// ------------------------------------------------------------- window tiles
+
+constexpr int kXtThreads = 1024;
+constexpr int kXtMaxWindow = 8192;
+
+template <typename T>
+struct XtView {
+  const int64_t* __restrict__ base;
+  const int32_t* __restrict__ off;
+  const uint32_t* __restrict__ idx;
+  const T* __restrict__ v;
+  int64_t nrows, ncols, windows;
+  int window;
+};
+
+// One workgroup per row group.  Per window: stage x's window in LDS
+// (coalesced), then every thread runs through its entries of the window
+// (its S rows, slot by slot, columns ascending) with the gathers served by
+// LDS.  A row's sum starts at 0 and adds a * x in column order with the
+// product and the sum rounded separately, the reference's
+// sparse_blas.hpp:26-31 arithmetic.  The slot of an entry picks its
+// accumulator through a select chain (adding 0 to the others is exact).
+template <typename T, int S, typename Epi>
+__global__ __launch_bounds__(kXtThreads) void xtile_spmv_kernel(
+    XtView<T> A, const T* __restrict__ x, Epi epi, double* partials) {
+#pragma clang fp contract(off)
+  __shared__ T win[kXtMaxWindow];
+  __shared__ double red[kXtThreads / 64];
+  if (epi.done && *epi.done) return;
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  const int64_t gbase = A.base[g];
+  const int32_t* __restrict__ off = A.off + g * (A.windows * kXtThreads + 1);
+  const uint32_t* __restrict__ idx = A.idx + gbase;
+  const T* __restrict__ val = A.v + gbase;
+  T acc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s] = T(0);
+  for (int64_t w = 0; w < A.windows; ++w) {
+    const int64_t c0 = w * A.window;
+    const int64_t left = A.ncols - c0;
+    const int wl = left < A.window ? (int)left : A.window;
+    const int k0 = off[w * kXtThreads + tid], k1 = off[w * kXtThreads + tid + 1];
+    __syncthreads();  // every thread is done with the previous window
+    for (int k = tid; k < wl; k += kXtThreads) win[k] = x[c0 + k];
+    __syncthreads();
+    for (int k = k0; k < k1; ++k) {
+      const uint32_t e = __builtin_nontemporal_load(idx + k);
+      const T a = __builtin_nontemporal_load(val + k);
+      const T prod = a * win[e & 0xFFFFu];
+      const int slot = (int)(e >> 16);
+#pragma unroll
+      for (int s = 0; s < S; ++s) acc[s] += slot == s ? prod : T(0);
+    }
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int64_t r = g * (int64_t)(kXtThreads * S) + tid + (int64_t)s * kXtThreads;
+    if (r < A.nrows) part += epi.row((int)r, acc[s]);
+  }
+  if constexpr (Epi::kReduce) {
+    const double t = block_sum_to_thread0<kXtThreads>(part, red);
+    if (tid == 0) partials[g] = t;
+  }
+}
+
+template <typename T, typename Epi>
+void launch_xtiles(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
+                   hipStream_t s) {
+  XtView<T> v{A.xt_base.p, A.xt_off.p, A.xt_idx.p, nullptr, A.nrows, A.ncols,
+              A.xt_windows, A.xt_window};
+  if constexpr (std::is_same<T, float>::value) v.v = A.tiles32();
+  else v.v = A.xt_v64.p;
+  const dim3 grid((unsigned)A.xt_groups), block(kXtThreads);
+  if (A.xt_slots == 16)
+    hipLaunchKernelGGL((xtile_spmv_kernel<T, 16, Epi>), grid, block, 0, s, v, x,
+                       epi, parts);
+  else
+    hipLaunchKernelGGL((xtile_spmv_kernel<T, 8, Epi>), grid, block, 0, s, v, x,
+                       epi, parts);
+  SPK_HIP_TRY(hipGetLastError());
+}
+
 template <typename T, typename Epi>
 void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
                      hipStream_t s) {
-  double* parts = Epi::kReduce ? reduce_partials(A.nblocks) : nullptr;
-  hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
-                     0, s, A.view<T>(), x, epi, parts, init);
-  SPK_HIP_TRY(hipGetLastError());
+  const int nparts = A.xt_groups ? (int)A.xt_groups : A.nblocks;
+  double* parts = Epi::kReduce ? reduce_partials(nparts) : nullptr;
+  if (A.xt_groups) {
+    launch_xtiles(A, x, epi, parts, s);
+  } else {
+    hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
+                       0, s, A.view<T>(), x, epi, parts, init);
+    SPK_HIP_TRY(hipGetLastError());
+  }
   if constexpr (Epi::kReduce) {
     hipLaunchKernelGGL((finish_kernel<1, EpiFinalize<Epi>>), dim3(1),
                        dim3(kFinishBlock), 0, s, (const double*)parts,
-                       A.nblocks, EpiFinalize<Epi>{epi}, epi.done);
+                       nparts, EpiFinalize<Epi>{epi}, epi.done);
     SPK_HIP_TRY(hipGetLastError());
   }
 }
@@ -330,6 +453,10 @@ template <typename T, typename Epi>
 int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
                          hipStream_t s) {
   static_assert(Epi::kReduce, "an epilogue with a sum");
+  if (A.xt_groups) {
+    launch_xtiles(A, x, epi, parts, s);
+    return (int)A.xt_groups;
+  }
   spk_csr_s& last = A.panels.empty() ? A : *A.panels.back();
   const T* init = nullptr;
   if (!A.panels.empty()) init = launch_panels(A, x, epi.done, s);
@@ -341,6 +468,7 @@ int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts
 
 // the number of partials launch_spmv_partials writes
 inline int spmv_partials_count(const spk_csr_s& A) {
+  if (A.xt_groups) return (int)A.xt_groups;
   return A.panels.empty() ? A.nblocks : A.panels.back()->nblocks;
 }
 

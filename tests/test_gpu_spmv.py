@@ -311,3 +311,64 @@ def test_spmv_dot_matches_oracle(O, spk):
     with pytest.raises(spk.InvalidArgument):
         B.spmv_dot(spk.DeviceVector.zeros(4), spk.DeviceVector.zeros(3),
                    spk.DeviceVector.zeros(1))
+
+
+@pytest.mark.parametrize("slots,window", [(16, 8192), (8, 8192), (16, 4096),
+                                          (8, 256)])
+def test_window_tiles_bit_exact(O, spk, slots, window):
+    """Window tiles sum every row in column order with separately rounded
+    products and sums: the oracle's storage-order multiply
+    (sparse_blas.hpp:26-31) bit for bit, FP64 and FP32, on the power-law
+    matrix (rows across all columns, 2.5 row groups) and on a rectangular
+    random matrix; the epilogue paths (<x, A x>, CG) run on it too."""
+    A = O.powerlaw_spd(40000, 6)
+    Ad = spk.CsrMatrix.from_oracle(A)
+    Ad.set_window_tiles(slots, window)
+    assert Ad.panels == -(slots * 65536 + window)
+    x = O.uniform_pm1(5, A.nrows)
+    y = spk.multiply(Ad, spk.DeviceVector.from_numpy(x)).numpy()
+    assert np.array_equal(y, O.spmv(A, x))
+    x32 = x.astype(np.float32)
+    y32 = spk.multiply(Ad, spk.DeviceVector.from_numpy(x32)).numpy()
+    assert np.array_equal(y32, O.spmv(A, x32))
+    yd = spk.DeviceVector.zeros(A.nrows)
+    r = spk.DeviceVector.zeros(1)
+    Ad.spmv_dot(spk.DeviceVector.from_numpy(x), yd, r)
+    assert np.array_equal(yd.numpy(), y)
+    assert abs(r.numpy()[0] - np.dot(x, y)) <= 1e-12 * np.abs(x * y).sum()
+    rng = np.random.default_rng(slots + window)
+    nr, nc = 20000, 70001
+    cols = [np.sort(rng.choice(nc, size=rng.integers(0, 60), replace=False))
+            for _ in range(nr)]
+    rp = np.concatenate([[0], np.cumsum([len(c) for c in cols])])
+    ci = np.concatenate(cols).astype(np.int64)
+    v = rng.uniform(-1, 1, len(ci))
+    R = O.Csr(nr, nc, rp, ci, v)
+    Rd = spk.CsrMatrix.from_host(nr, nc, rp, ci, v)
+    Rd.set_window_tiles(slots, window)
+    xr = O.uniform_pm1(8, nc)
+    assert np.array_equal(spk.multiply(Rd, spk.DeviceVector.from_numpy(xr)).numpy(),
+                          O.spmv(R, xr))
+
+
+def test_window_tiles_in_solvers_and_layout_switches(O, spk):
+    A = spk.powerlaw_spd(60000, 3)
+    b = spk.uniform_pm1(9, A.nrows)
+    its = []
+    for code in (1, -(16 * 65536 + 8192), 8):
+        A.set_layout(code)
+        assert A.panels == code
+        x = spk.DeviceVector.zeros(A.nrows)
+        s = spk.conjugate_gradient(b, x, spk.CGConfig(1e-10, 1000, 1000), A,
+                                   spk.jacobi(A))
+        assert s.converged and spk.true_relres(A, b, x) < 1e-9
+        its.append(s.computed_iterations)
+        x = spk.DeviceVector.zeros(A.nrows)
+        s = spk.bicgstab(b, x, spk.BicgstabConfig(1e-10, 1000), A, spk.jacobi(A))
+        assert s.converged
+    assert max(its) - min(its) <= 1
+    # a row whose columns do not ascend is rejected (the store is unchanged)
+    U = spk.CsrMatrix.from_host(2, 3, [0, 2, 3], [2, 0, 1], [1.0, 2.0, 3.0])
+    with pytest.raises(spk.InvalidArgument, match="ascending"):
+        U.set_window_tiles(8, 256)
+    assert U.panels == 1
