@@ -48,12 +48,13 @@ HBM_PEAK_GBS = 8000.0
 # W1[24][64] b1[64] W2[64][64] b2[64] W3[64][9] b3[9]
 SPAI_W3 = 24 * 64 + 64 + 64 * 64 + 64
 SPAI_B3 = SPAI_W3 + 64 * 9
-# HBM-side bytes per SpMV measured with rocprofv3 PMC counters (FETCH_SIZE x
-# 2.0 + WRITE_SIZE, calibrated on a 256 MiB copy) for the default
-# configurations; the commands and the raw counts are in
-# profiles/spmv_traffic_pmc.md
-MEASURED_TRAFFIC = {"3D 7-point Poisson 192^3": 800.6e6,
-                    "power-law SPD n=4194304": 107.2e9}
+# HBM-side bytes per call of the roofline kernel (the SpMV with the <p,q>
+# epilogue) measured with rocprofv3 PMC counters (FETCH_SIZE x 2.0 +
+# WRITE_SIZE, calibrated on a 256 MiB copy) at the layouts the tuner picks
+# (plain CSR for the Poisson matrix, 16 column panels for the power-law
+# one); the commands and the raw counts are in profiles/spmv_traffic_pmc.md
+MEASURED_TRAFFIC = {"3D 7-point Poisson 192^3": 792.9e6,
+                    "power-law SPD n=4194304": 12.39e9}
 METRIC = ("time-to-1e-8 residual: learned-FGMRES vs Jacobi-PCG; "
           "SpMV GB/s % HBM peak")
 

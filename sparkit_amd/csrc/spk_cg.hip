@@ -473,6 +473,12 @@ void require_ready(spk_csr_s& S, bool fp32) {
   const char* msgp =
       "no-allocation solve: a column-panel store's running-sum vector is not "
       "made yet (run one SpMV with it first)";
+  if (S.xt_groups) {
+    // window tiles: the tiles' own FP32 values and the long rows' store
+    SPK_REQUIRE(!fp32 || (S.xt_has32 && (!S.xt_long || S.xt_long->has32)),
+                SPK_INVALID_ARGUMENT, msg32);
+    return;
+  }
   SPK_REQUIRE(!fp32 || S.has32, SPK_INVALID_ARGUMENT, msg32);
   if (!S.panels.empty()) {
     SPK_REQUIRE(fp32 ? S.pacc32.n >= (size_t)S.nrows : S.pacc64.n >= (size_t)S.nrows,
@@ -482,6 +488,7 @@ void require_ready(spk_csr_s& S, bool fp32) {
 }
 
 size_t max_blocks(const spk_csr_s& S) {
+  if (S.xt_groups) return (size_t)xtile_partials_count(S);
   size_t m = (size_t)S.nblocks;
   for (const spk_csr_s* q : S.panels) m = std::max(m, (size_t)q->nblocks);
   return m;

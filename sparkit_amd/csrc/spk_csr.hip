@@ -157,29 +157,37 @@ __global__ void panel_copy_kernel(int64_t n, const int32_t* __restrict__ start,
 }
 
 // Window tiles: one thread per (group, owner thread) walks its S rows'
-// cursors window by window.  COUNT: writes each (window, thread) segment's
-// length into off and flags rows whose columns do not ascend; FILL: writes
-// the entries at base[g] + off[window * T + t] (off scanned by then).
+// cursors window by window; rows longer than lmax are skipped.  COUNT:
+// writes each (window, thread) segment's length into off, the owner's
+// long-row slots into skip, and flags rows whose columns do not ascend;
+// FILL: writes the entries at base[g] + off[window * T + t] (off scanned by
+// then).
 template <int S, bool FILL>
 __global__ void xt_walk_kernel(int64_t nrows, const int32_t* __restrict__ rp,
                                const int32_t* __restrict__ ci,
                                const double* __restrict__ v, int64_t groups,
-                               int64_t windows, int window,
+                               int64_t windows, int window, int lmax,
                                int32_t* __restrict__ off,
                                const int64_t* __restrict__ base,
                                uint32_t* __restrict__ idx, double* __restrict__ xv,
-                               int* __restrict__ bad) {
+                               uint32_t* __restrict__ skip, int* __restrict__ bad) {
   const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (gt >= groups * kXtThreads) return;
   const int64_t g = gt / kXtThreads;
   const int t = (int)(gt % kXtThreads);
   int cur[S], end[S];
+  uint32_t mask = 0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int64_t r = g * (int64_t)(kXtThreads * S) + t + (int64_t)s * kXtThreads;
     cur[s] = r < nrows ? rp[r] : 0;
     end[s] = r < nrows ? rp[r + 1] : 0;
+    if (end[s] - cur[s] > lmax) {
+      mask |= 1u << s;
+      cur[s] = end[s];
+    }
   }
+  if (!FILL) skip[gt] = mask;
   int32_t* o = off + g * (windows * kXtThreads + 1);
   for (int64_t w = 0; w < windows; ++w) {
     const int64_t c0 = w * (int64_t)window, limit = c0 + window;
@@ -206,7 +214,7 @@ __global__ void xt_walk_kernel(int64_t nrows, const int32_t* __restrict__ rp,
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int64_t r = g * (int64_t)(kXtThreads * S) + t + (int64_t)s * kXtThreads;
-      if (r >= nrows) continue;
+      if (r >= nrows || ((mask >> s) & 1u)) continue;
       for (int k = rp[r] + 1; k < rp[r + 1]; ++k)
         if (ci[k] <= ci[k - 1]) atomicOr(bad, 1);
       if (cur[s] != end[s]) atomicOr(bad, 2);  // a column beyond the last window
@@ -251,6 +259,63 @@ __global__ void xt_totals_kernel(const int32_t* __restrict__ off, int64_t N,
   if (g < groups) tot[g] = off[g * (N + 1) + N];
 }
 
+// rows[i]'s entries of (rp, ci, v) to position qrp[i] of (qci, qv): one
+// wavefront per row
+__global__ void rows_copy_kernel(int64_t m, const int32_t* __restrict__ rows,
+                                 const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci,
+                                 const double* __restrict__ v,
+                                 const int32_t* __restrict__ qrp,
+                                 int32_t* __restrict__ qci, double* __restrict__ qv) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (i >= m) return;
+  const int a = rp[rows[i]], b = rp[rows[i] + 1];
+  const int o = qrp[i] - a;
+  for (int k = a + lane; k < b; k += 64) {
+    qci[k + o] = ci[k];
+    qv[k + o] = v[k];
+  }
+}
+
+// The long rows' compressed store (rows with more than lmax entries; none:
+// null), with its row map and block plan.
+spk_csr_s* build_long_rows(spk_csr_s& A, int lmax) {
+  std::vector<int32_t> lrow, lrp{0};
+  for (int64_t i = 0; i < A.nrows; ++i) {
+    const int32_t len = A.h_rp[(size_t)i + 1] - A.h_rp[(size_t)i];
+    if (len > lmax) {
+      lrow.push_back((int32_t)i);
+      lrp.push_back(lrp.back() + len);
+    }
+  }
+  if (lrow.empty()) return nullptr;
+  hipStream_t s = stream();
+  const int64_t m = (int64_t)lrow.size();
+  auto* Q = new spk_csr_s();
+  try {
+    Q->allocate(m, A.ncols, lrp.back());
+    Q->rows.alloc((size_t)m);
+    SPK_HIP_TRY(hipMemcpyAsync(Q->rp.p, lrp.data(), (m + 1) * 4,
+                               hipMemcpyHostToDevice, s));
+    SPK_HIP_TRY(hipMemcpyAsync(Q->rows.p, lrow.data(), m * 4,
+                               hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(rows_copy_kernel, dim3((unsigned)((m * 64 + 255) / 256)),
+                       dim3(256), 0, s, m, (const int32_t*)Q->rows.p,
+                       (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                       (const double*)A.v64.p, (const int32_t*)Q->rp.p, Q->ci.p,
+                       Q->v64.p);
+    SPK_HIP_TRY(hipGetLastError());
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    Q->h_rp = std::move(lrp);
+    upload_plan(*Q, build_spmv_plan(Q->h_rp.data(), m, &Q->nlong));
+  } catch (...) {
+    delete Q;
+    throw;
+  }
+  return Q;
+}
+
 }  // namespace
 
 void build_tiles(spk_csr_s& A, int S, int window) {
@@ -267,35 +332,42 @@ void build_tiles(spk_csr_s& A, int S, int window) {
   const int64_t N = windows * kXtThreads;
   SPK_REQUIRE(N < ((int64_t)1 << 31) / 2, SPK_INVALID_ARGUMENT,
               "window tiles: too many windows");
+  // long rows would hold up their owner's wavefront in every window
+  const int lmax = (int)std::max<int64_t>(256, 8 * windows);
+  int64_t long_nnz = 0;
+  for (int64_t i = 0; i < A.nrows; ++i) {
+    const int32_t len = A.h_rp[(size_t)i + 1] - A.h_rp[(size_t)i];
+    if (len > lmax) long_nnz += len;
+  }
   DevBuf<int32_t> off((size_t)(groups * (N + 1)));
+  DevBuf<uint32_t> skip((size_t)(groups * kXtThreads));
   DevBuf<int> bad(1);
   DevBuf<int64_t> tot((size_t)groups);
   SPK_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
   const int64_t threads = groups * kXtThreads;
   const unsigned grid = (unsigned)((threads + 255) / 256);
   auto walk = [&](bool fill, const int64_t* base, uint32_t* idx, double* xv) {
+    const int32_t* rp = A.rp.p;
+    const int32_t* ci = A.ci.p;
+    const double* v = A.v64.p;
     if (S == 16) {
       if (fill)
         hipLaunchKernelGGL((xt_walk_kernel<16, true>), dim3(grid), dim3(256), 0, s,
-                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
-                           (const double*)A.v64.p, groups, windows, window, off.p,
-                           base, idx, xv, bad.p);
+                           A.nrows, rp, ci, v, groups, windows, window, lmax, off.p,
+                           base, idx, xv, skip.p, bad.p);
       else
         hipLaunchKernelGGL((xt_walk_kernel<16, false>), dim3(grid), dim3(256), 0, s,
-                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
-                           (const double*)A.v64.p, groups, windows, window, off.p,
-                           base, idx, xv, bad.p);
+                           A.nrows, rp, ci, v, groups, windows, window, lmax, off.p,
+                           base, idx, xv, skip.p, bad.p);
     } else {
       if (fill)
         hipLaunchKernelGGL((xt_walk_kernel<8, true>), dim3(grid), dim3(256), 0, s,
-                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
-                           (const double*)A.v64.p, groups, windows, window, off.p,
-                           base, idx, xv, bad.p);
+                           A.nrows, rp, ci, v, groups, windows, window, lmax, off.p,
+                           base, idx, xv, skip.p, bad.p);
       else
         hipLaunchKernelGGL((xt_walk_kernel<8, false>), dim3(grid), dim3(256), 0, s,
-                           A.nrows, (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
-                           (const double*)A.v64.p, groups, windows, window, off.p,
-                           base, idx, xv, bad.p);
+                           A.nrows, rp, ci, v, groups, windows, window, lmax, off.p,
+                           base, idx, xv, skip.p, bad.p);
     }
     SPK_HIP_TRY(hipGetLastError());
   };
@@ -314,16 +386,20 @@ void build_tiles(spk_csr_s& A, int S, int window) {
   SPK_HIP_TRY(hipStreamSynchronize(s));
   hbase[0] = 0;
   for (int64_t g = 0; g < groups; ++g) hbase[(size_t)g + 1] = hbase[(size_t)g] + htot[(size_t)g];
-  SPK_REQUIRE(hbase[(size_t)groups] == A.nnz, SPK_HIP_ERROR,
+  SPK_REQUIRE(hbase[(size_t)groups] == A.nnz - long_nnz, SPK_HIP_ERROR,
               "window tiles: entry count mismatch");
+  A.xt_long = build_long_rows(A, lmax);
   A.xt_base.alloc((size_t)groups + 1);
   SPK_HIP_TRY(hipMemcpyAsync(A.xt_base.p, hbase.data(), (groups + 1) * 8,
                              hipMemcpyHostToDevice, s));
-  A.xt_idx.alloc((size_t)A.nnz);
-  A.xt_v64.alloc((size_t)A.nnz);
+  const size_t ntile = (size_t)std::max<int64_t>(hbase[(size_t)groups], 1);
+  A.xt_idx.alloc(ntile);
+  A.xt_v64.alloc(ntile);
   walk(true, A.xt_base.p, A.xt_idx.p, A.xt_v64.p);
   SPK_HIP_TRY(hipStreamSynchronize(s));
   A.xt_off = std::move(off);
+  A.xt_skip = std::move(skip);
+  A.xt_lmax = lmax;
   A.xt_slots = S;
   A.xt_window = window;
   A.xt_groups = groups;
@@ -420,10 +496,13 @@ void spk_csr_s::allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_) {
 
 const float* spk_csr_s::tiles32() {
   if (!xt_has32) {
-    xt_v32.alloc((size_t)(nnz > 0 ? nnz : 1));
-    if (nnz > 0)
-      hipLaunchKernelGGL(spk::f64_to_f32_kernel, dim3(spk::grid_for(nnz)),
-                         dim3(256), 0, spk::stream(), xt_v64.p, xt_v32.p, nnz);
+    // the tiles hold the entries of the tiled rows only (nnz minus the long
+    // rows' entries): convert what xt_v64 holds
+    const int64_t m = (int64_t)xt_v64.n;
+    xt_v32.alloc((size_t)m);
+    if (m > 0)
+      hipLaunchKernelGGL(spk::f64_to_f32_kernel, dim3(spk::grid_for(m)),
+                         dim3(256), 0, spk::stream(), xt_v64.p, xt_v32.p, m);
     SPK_HIP_TRY(hipGetLastError());
     xt_has32 = true;
   }

@@ -78,16 +78,22 @@ struct spk_csr_s {
   // thread t of a group owns its rows t + s * kXtThreads (slot s).  For
   // every window of xt_window columns each thread's entries of its rows in
   // that window are stored together, in (slot, column) order, as
-  // (slot << 16 | column - window start, value); the SpMV stages the x
-  // window in LDS, so every gather is an LDS read, and a row's sum runs in
-  // column order (the CSR storage order when rows are sorted).
-  int xt_slots = 0, xt_window = 0;
+  // (slot << 16 | column - window start, value), and a window's segments
+  // follow each other in thread order; the SpMV stages the x window and the
+  // window's entries in LDS with coalesced loads, so every gather is an LDS
+  // read, and a row's sum runs in column order (the CSR storage order when
+  // rows are sorted).  Rows longer than xt_lmax stay out of the tiles (their
+  // owner's bit in xt_skip is set): they form the compressed store xt_long,
+  // run by the row kernel after the tiles.
+  int xt_slots = 0, xt_window = 0, xt_lmax = 0;
   int64_t xt_groups = 0, xt_windows = 0;
   spk::DevBuf<int64_t> xt_base;  // first entry of each group (groups + 1)
   spk::DevBuf<int32_t> xt_off;   // per group, (windows * threads + 1) offsets
   spk::DevBuf<uint32_t> xt_idx;
   spk::DevBuf<double> xt_v64;
   spk::DevBuf<float> xt_v32;
+  spk::DevBuf<uint32_t> xt_skip;  // per (group, thread): slots of long rows
+  spk_csr_s* xt_long = nullptr;
   bool xt_has32 = false;
 
   spk_csr_s() = default;
@@ -100,13 +106,16 @@ struct spk_csr_s {
     drop_tiles();
   }
   void drop_tiles() {
-    xt_slots = xt_window = 0;
+    xt_slots = xt_window = xt_lmax = 0;
     xt_groups = xt_windows = 0;
     xt_base.release();
     xt_off.release();
     xt_idx.release();
     xt_v64.release();
     xt_v32.release();
+    xt_skip.release();
+    delete xt_long;
+    xt_long = nullptr;
     xt_has32 = false;
   }
   const float* tiles32();
@@ -152,8 +161,9 @@ void finish_store(spk_csr_s& A, bool validate_cols);
 // Split A into `panels` column panels (1: none); values are copied.
 void build_panels(spk_csr_s& A, int panels);
 // Window tiles with `slots` rows per thread (8 or 16) and `window` columns
-// per x window (<= kXtMaxWindow); drops panels.  SPK_INVALID_ARGUMENT if a
-// row's columns are not ascending.
+// per x window (<= kXtMaxWindow); drops panels.  Rows longer than
+// max(256, 8 * windows) go to the long-row store.  SPK_INVALID_ARGUMENT if a
+// tiled row's columns are not ascending.
 void build_tiles(spk_csr_s& A, int slots, int window);
 
 // ------------------------------------------------------------- epilogues
@@ -325,6 +335,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
 
 constexpr int kXtThreads = 1024;
 constexpr int kXtMaxWindow = 8192;
+constexpr int kXtChunk = 7168;  // entries staged in LDS per pass
 
 template <typename T>
 struct XtView {
@@ -332,55 +343,123 @@ struct XtView {
   const int32_t* __restrict__ off;
   const uint32_t* __restrict__ idx;
   const T* __restrict__ v;
+  const uint32_t* __restrict__ skip;
   int64_t nrows, ncols, windows;
   int window;
 };
 
-// One workgroup per row group.  Per window: stage x's window in LDS
-// (coalesced), then every thread runs through its entries of the window
-// (its S rows, slot by slot, columns ascending) with the gathers served by
-// LDS.  A row's sum starts at 0 and adds a * x in column order with the
-// product and the sum rounded separately, the reference's
-// sparse_blas.hpp:26-31 arithmetic.  The slot of an entry picks its
-// accumulator through a select chain (adding 0 to the others is exact).
+// One workgroup per row group.  Per window: x's window and the window's
+// entries (up to kXtChunk per pass) go to LDS with coalesced loads, then
+// every thread runs through its own entries of the window (its S rows, slot
+// by slot, columns ascending) with both served by LDS.  The loads of the
+// next window's x, first entry chunk and thread ranges are issued into
+// registers before the current window's arithmetic, and that arithmetic
+// reads only registers and LDS, so the loads are in flight while it runs;
+// the window's entry range reaches every thread through LDS.  A row's sum
+// starts at 0 and adds a * x in column order with the product and the sum
+// rounded separately, the reference's sparse_blas.hpp:26-31 arithmetic.  A
+// thread's entries of a window come slot by slot, so it walks the slots in
+// order and adds each run to its slot's accumulator.  Rows marked in `skip`
+// (long rows) are left to the row kernel.
 template <typename T, int S, typename Epi>
 __global__ __launch_bounds__(kXtThreads) void xtile_spmv_kernel(
     XtView<T> A, const T* __restrict__ x, Epi epi, double* partials) {
 #pragma clang fp contract(off)
+  constexpr int XJ = kXtMaxWindow / kXtThreads;
+  constexpr int EJ = kXtChunk / kXtThreads;
+  static_assert(XJ * kXtThreads == kXtMaxWindow && EJ * kXtThreads == kXtChunk,
+                "staging divides evenly");
   __shared__ T win[kXtMaxWindow];
+  __shared__ T eval[kXtChunk];
+  __shared__ uint32_t eidx[kXtChunk];
   __shared__ double red[kXtThreads / 64];
+  __shared__ int range[2];
   if (epi.done && *epi.done) return;
   const int tid = threadIdx.x;
   const int64_t g = blockIdx.x;
   const int64_t gbase = A.base[g];
+  const int gtot = (int)(A.base[g + 1] - gbase);  // the group's entries
   const int32_t* __restrict__ off = A.off + g * (A.windows * kXtThreads + 1);
   const uint32_t* __restrict__ idx = A.idx + gbase;
   const T* __restrict__ val = A.v + gbase;
   T acc[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) acc[s] = T(0);
-  for (int64_t w = 0; w < A.windows; ++w) {
+  // the next window's x, thread range and first entry chunk (the entries
+  // from the window's start on, up to kXtChunk of them), in registers
+  T xr[XJ], vr[EJ];
+  uint32_t ir[EJ];
+  int nk0 = 0, nk1 = 0;
+  auto fetch = [&](int64_t w, int nb) {
     const int64_t c0 = w * A.window;
     const int64_t left = A.ncols - c0;
     const int wl = left < A.window ? (int)left : A.window;
-    const int k0 = off[w * kXtThreads + tid], k1 = off[w * kXtThreads + tid + 1];
-    __syncthreads();  // every thread is done with the previous window
-    for (int k = tid; k < wl; k += kXtThreads) win[k] = x[c0 + k];
-    __syncthreads();
-    for (int k = k0; k < k1; ++k) {
-      const uint32_t e = __builtin_nontemporal_load(idx + k);
-      const T a = __builtin_nontemporal_load(val + k);
-      const T prod = a * win[e & 0xFFFFu];
-      const int slot = (int)(e >> 16);
+    nk0 = off[w * kXtThreads + tid];
+    nk1 = off[w * kXtThreads + tid + 1];
+    const int cend = gtot - nb < kXtChunk ? gtot : nb + kXtChunk;
 #pragma unroll
-      for (int s = 0; s < S; ++s) acc[s] += slot == s ? prod : T(0);
+    for (int j = 0; j < XJ; ++j) {
+      const int k = tid + j * kXtThreads;
+      xr[j] = k < wl ? x[c0 + k] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < EJ; ++j) {
+      const int k = nb + tid + j * kXtThreads;
+      const bool in = k < cend;
+      ir[j] = in ? __builtin_nontemporal_load(idx + k) : 0u;
+      vr[j] = in ? __builtin_nontemporal_load(val + k) : T(0);
+    }
+  };
+  fetch(0, 0);
+  for (int64_t w = 0; w < A.windows; ++w) {
+    const int k0 = nk0, k1 = nk1;
+    __syncthreads();  // every thread is done with the previous window
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) win[tid + j * kXtThreads] = xr[j];
+#pragma unroll
+    for (int j = 0; j < EJ; ++j) {
+      eidx[tid + j * kXtThreads] = ir[j];
+      eval[tid + j * kXtThreads] = vr[j];
+    }
+    if (tid == 0) range[0] = k0;
+    if (tid == kXtThreads - 1) range[1] = k1;
+    __syncthreads();
+    const int wb = range[0], we = range[1];  // block-uniform
+    if (w + 1 < A.windows) fetch(w + 1, we);
+    if (we > wb) {
+      int cb = wb;
+      for (;;) {
+        const int ce = we - cb < kXtChunk ? we : cb + kXtChunk;
+        const int a1 = k1 < ce ? k1 : ce;
+        int k = k0 > cb ? k0 : cb;
+        uint32_t e = k < a1 ? eidx[k - cb] : 0xFFFFFFFFu;  // slot 0xFFFF: none
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          while ((e >> 16) == (uint32_t)s) {
+            acc[s] += eval[k - cb] * win[e & 0xFFFFu];
+            ++k;
+            e = k < a1 ? eidx[k - cb] : 0xFFFFFFFFu;
+          }
+        }
+        if (ce == we) break;
+        // a further chunk of this window (rare: the window is unusually full)
+        cb = ce;
+        const int ce2 = we - cb < kXtChunk ? we : cb + kXtChunk;
+        __syncthreads();
+        for (int kk = cb + tid; kk < ce2; kk += kXtThreads) {
+          eidx[kk - cb] = __builtin_nontemporal_load(idx + kk);
+          eval[kk - cb] = __builtin_nontemporal_load(val + kk);
+        }
+        __syncthreads();
+      }
     }
   }
+  const uint32_t skip = A.skip[g * kXtThreads + tid];
   double part = 0.0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int64_t r = g * (int64_t)(kXtThreads * S) + tid + (int64_t)s * kXtThreads;
-    if (r < A.nrows) part += epi.row((int)r, acc[s]);
+    if (r < A.nrows && !((skip >> s) & 1u)) part += epi.row((int)r, acc[s]);
   }
   if constexpr (Epi::kReduce) {
     const double t = block_sum_to_thread0<kXtThreads>(part, red);
@@ -388,11 +467,17 @@ __global__ __launch_bounds__(kXtThreads) void xtile_spmv_kernel(
   }
 }
 
+// the number of partials a tiled SpMV writes: one per group, then the long
+// rows' blocks
+inline int xtile_partials_count(const spk_csr_s& A) {
+  return (int)A.xt_groups + (A.xt_long ? A.xt_long->nblocks : 0);
+}
+
 template <typename T, typename Epi>
 void launch_xtiles(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
                    hipStream_t s) {
-  XtView<T> v{A.xt_base.p, A.xt_off.p, A.xt_idx.p, nullptr, A.nrows, A.ncols,
-              A.xt_windows, A.xt_window};
+  XtView<T> v{A.xt_base.p, A.xt_off.p, A.xt_idx.p, nullptr, A.xt_skip.p,
+              A.nrows, A.ncols, A.xt_windows, A.xt_window};
   if constexpr (std::is_same<T, float>::value) v.v = A.tiles32();
   else v.v = A.xt_v64.p;
   const dim3 grid((unsigned)A.xt_groups), block(kXtThreads);
@@ -403,12 +488,20 @@ void launch_xtiles(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
     hipLaunchKernelGGL((xtile_spmv_kernel<T, 8, Epi>), grid, block, 0, s, v, x,
                        epi, parts);
   SPK_HIP_TRY(hipGetLastError());
+  if (A.xt_long && A.xt_long->nblocks > 0) {
+    // the long rows: their store maps each stored row to its matrix row
+    spk_csr_s& L = *A.xt_long;
+    hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(L.nblocks), dim3(kSpmvBlock), 0,
+                       s, L.view<T>(), x, epi, parts ? parts + A.xt_groups : nullptr,
+                       (const T*)nullptr);
+    SPK_HIP_TRY(hipGetLastError());
+  }
 }
 
 template <typename T, typename Epi>
 void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
                      hipStream_t s) {
-  const int nparts = A.xt_groups ? (int)A.xt_groups : A.nblocks;
+  const int nparts = A.xt_groups ? xtile_partials_count(A) : A.nblocks;
   double* parts = Epi::kReduce ? reduce_partials(nparts) : nullptr;
   if (A.xt_groups) {
     launch_xtiles(A, x, epi, parts, s);
@@ -455,7 +548,7 @@ int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts
   static_assert(Epi::kReduce, "an epilogue with a sum");
   if (A.xt_groups) {
     launch_xtiles(A, x, epi, parts, s);
-    return (int)A.xt_groups;
+    return xtile_partials_count(A);
   }
   spk_csr_s& last = A.panels.empty() ? A : *A.panels.back();
   const T* init = nullptr;
@@ -468,7 +561,7 @@ int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts
 
 // the number of partials launch_spmv_partials writes
 inline int spmv_partials_count(const spk_csr_s& A) {
-  if (A.xt_groups) return (int)A.xt_groups;
+  if (A.xt_groups) return xtile_partials_count(A);
   return A.panels.empty() ? A.nblocks : A.panels.back()->nblocks;
 }
 

@@ -62,6 +62,26 @@ def test_noalloc_cg_fp32_and_explicit_inverse(spk):
     _same(r1, r2)
 
 
+def test_noalloc_cg_on_window_tiles(spk):
+    """A window-tile store (its long rows in a store of their own): FP64, and
+    FP32 once an ordinary FP32 solve has made the tiles' FP32 values."""
+    print("no-alloc CG: power-law 40000 on 8x256 window tiles", flush=True)
+    A = spk.powerlaw_spd(40000, 5)
+    A.set_window_tiles(8, 256)
+    J = spk.jacobi(A)
+    b = spk.uniform_pm1(3, A.nrows)
+    r1, r2 = _pair(spk, A, J, b, spk.CGConfig(1e-10, 2000, 2000, True, spk.F64))
+    assert r1[1].converged
+    _same(r1, r2)
+    b32 = b.astype(np.float32)
+    cfg32 = spk.CGConfig(1e-5, 2000, 2000, False, spk.F32)
+    x32 = spk.DeviceVector.zeros(A.nrows, np.float32)
+    with pytest.raises(spk.InvalidArgument, match="FP32 copy"):
+        spk.conjugate_gradient_noalloc(b32, x32, cfg32, A, J)
+    r1, r2 = _pair(spk, A, J, b32, cfg32)  # the first call makes the copies
+    _same(r1, r2)
+
+
 def test_noalloc_cg_takes_no_device_memory(spk):
     """A size no ordinary solve of this process has used: the ordinary path
     would grow its workspace; the no-allocation path leaves the device's
