@@ -528,6 +528,19 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
             inner[name] = P
             res[name]["inner_iterations_per_solve"] = \
                 P.inner_iterations // 2  # warm-up + timed solve
+    # the flexible slot holding FP32 PCG with FP64 reliable updates and the
+    # learned inverse's symmetric part (spk_precond_create_inner_ir), each
+    # application to a relative tolerance: 1e-4 (FGMRES takes a few outer
+    # steps) or 1e-8 (one outer step; FGMRES checks the true residual)
+    for tol in (1e-4, 1e-8):
+        icfg = spk.IrConfig(tolerance=tol, max_outer=200, inner_tolerance=1e-1,
+                            restart=0, max_inner=50000,
+                            inner_method=spk.IR_CG_RELIABLE)
+        P = spk.inner_ir(A, Ms, icfg)
+        name = f"fgmres30_inner_relcg_to_{tol:g}_learned_fp32"
+        run(name, lambda x: spk.fgmres(b, x, fg, A, P))
+        inner[name] = P
+        res[name]["inner_iterations_per_solve"] = P.inner_iterations // 2
     guarded("guarded_fgmres30_learned", Ml, False)  # FGMRES above: warm
     poisoned = spk.SpaiModel(1)
     pp = model.params
@@ -543,7 +556,15 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                if res[k].get("converged") and res[k]["true_relres"] < 1.01e-8)
     res["learned_fgmres_vs_jacobi_pcg"] = {
         "solve": best[1], "ratio": speedup(res, "jacobi_pcg_fp64", best[1]),
-        "target": 2.0}
+        "outer_iterations": res[best[1]]["iterations"], "target": 2.0}
+    # the same among the forms where FGMRES itself takes two or more steps
+    multi = min((res[k]["ms"], k) for k in
+                ["fgmres30_learned_fp64"] + [k for k in inner if "learned" in k]
+                if res[k].get("converged") and res[k]["true_relres"] < 1.01e-8
+                and res[k]["iterations"] >= 2)
+    res["learned_fgmres_vs_jacobi_pcg_two_or_more_outer"] = {
+        "solve": multi[1], "ratio": speedup(res, "jacobi_pcg_fp64", multi[1]),
+        "outer_iterations": res[multi[1]]["iterations"]}
     res["fallback_overhead_vs_jacobi_pcg"] = speedup(
         res, "jacobi_pcg_fp64", "guarded_fgmres30_nan_network")
     return res

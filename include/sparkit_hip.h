@@ -290,7 +290,8 @@ spk_status spk_precond_create_spmv(spk_csr M, spk_precond* out);
 spk_status spk_precond_create_inner_cg(spk_csr A, spk_precond M_inner,
                                        int64_t iterations, double tolerance,
                                        int precision, spk_precond* out);
-/* PCG steps run by all applications of an inner-PCG preconditioner */
+/* Inner steps run by all applications of an inner-PCG preconditioner (or of
+ * an inner mixed-precision solve, spk_precond_create_inner_ir) */
 spk_status spk_precond_inner_cg_iterations(spk_precond M, int64_t* total);
 spk_status spk_precond_destroy(spk_precond M);
 /* jacobi_apply (jacobi.hpp:42-50) / ilu_apply (ilu.hpp:173-179) / identity
@@ -466,6 +467,19 @@ typedef struct {
 spk_status spk_ir_solve(spk_csr A, spk_precond M, const spk_ir_config* cfg,
                         const double* d_b, double* d_x, spk_ir_summary* out,
                         double* history);
+
+/* A preconditioner for fgmres's flexible slot (fgmres.hpp:246-256; no
+ * reference counterpart) that runs a mixed-precision solve: z = the result
+ * of spk_ir_solve(A, M_inner, cfg) on A z = r from z = 0, i.e. z reaches
+ * ||r - A z|| < cfg->tolerance ||r|| (FP64 true residual) or cfg's budgets.
+ * inner_method is SPK_IR_CG, SPK_IR_CG_RELIABLE or SPK_IR_BICGSTAB (an
+ * inner (F)GMRES would share the outer one's work vectors); r and z are
+ * FP64 at the boundary (converted for an FP32 caller).  Not a fixed
+ * operator: conjugate_gradient and spk_ir_solve reject it.  A and M_inner
+ * must outlive it; spk_precond_inner_cg_iterations counts its inner steps. */
+spk_status spk_precond_create_inner_ir(spk_csr A, spk_precond M_inner,
+                                       const spk_ir_config* cfg,
+                                       spk_precond* out);
 
 /* ||b - A x||_2 / ||b||_2 in FP64 (the true-residual check). */
 spk_status spk_true_relres(spk_csr A, const double* d_b, const double* d_x,

@@ -38,7 +38,7 @@ __all__ = [
     "GmresConfig", "FgmresConfig", "BicgstabConfig", "Summary",
     "conjugate_gradient", "conjugate_gradient_noalloc", "gmres", "fgmres", "bicgstab", "cgs", "CgsConfig",
     "bicg", "BicgConfig", "minres", "MinresConfig", "qmr", "QmrConfig",
-    "true_relres", "dot", "axpby", "inner_cg", "InnerCg", "learned_solve", "LearnedConfig",
+    "true_relres", "dot", "axpby", "inner_cg", "inner_ir", "InnerCg", "learned_solve", "LearnedConfig",
     "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
     "FALLBACK_ILU_GMRES", "FALLBACK_NONE",
     "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
@@ -446,12 +446,13 @@ def ilu_apply(F: Ilu0, r: DeviceVector, z: DeviceVector):
 
 
 class InnerCg(Preconditioner):
-    """FGMRES's flexible-slot preconditioner: up to `iterations` PCG steps
-    on A z = r with M_inner, in `precision`."""
+    """FGMRES's flexible-slot preconditioner running a solve of its own:
+    up to `iterations` PCG steps on A z = r with M_inner, in `precision`
+    (inner_cg), or a mixed-precision solve (inner_ir)."""
 
     @property
     def inner_iterations(self) -> int:
-        """PCG steps run by all applications so far."""
+        """Inner steps run by all applications so far."""
         t = C.c_int64()
         call("spk_precond_inner_cg_iterations", self.h, C.byref(t))
         return t.value
@@ -852,16 +853,36 @@ def ir_solve(b: DeviceVector, x: DeviceVector, cfg: IrConfig, A: CsrMatrix,
     """x (FP64, updated in place) from b (FP64) by iterative refinement."""
     if b.dtype != np.float64 or x.dtype != np.float64:
         raise InvalidArgument(1, "ir_solve", "b and x must be float64")
-    c = _lib.IrConfig(float(cfg.tolerance), int(cfg.max_outer),
-                      float(cfg.inner_tolerance), int(cfg.restart),
-                      int(cfg.max_inner), int(cfg.inner_method),
-                      int(cfg.orthogonalization))
+    c = _ir_struct(cfg)
     s = _lib.IrSummary()
     hist = np.zeros(cfg.max_outer + 2, np.float64)
     call("spk_ir_solve", A.h, _h(M), C.byref(c), b.ptr, x.ptr, C.byref(s),
          hist.ctypes.data_as(C.c_void_p))
     return IrSummary(s.relres, s.outer_iterations, s.inner_iterations,
                      bool(s.converged), hist[:s.outer_iterations + 1].tolist())
+
+
+def _ir_struct(cfg: IrConfig):
+    return _lib.IrConfig(float(cfg.tolerance), int(cfg.max_outer),
+                         float(cfg.inner_tolerance), int(cfg.restart),
+                         int(cfg.max_inner), int(cfg.inner_method),
+                         int(cfg.orthogonalization))
+
+
+def inner_ir(A: CsrMatrix, M_inner=None, cfg: IrConfig = None) -> InnerCg:
+    """FGMRES's flexible-slot preconditioner running a mixed-precision
+    solve (spk_precond_create_inner_ir): z = ir_solve(A z = r) from z = 0
+    with M_inner; inner_method IR_CG, IR_CG_RELIABLE or IR_BICGSTAB.
+    inner_iterations counts the FP32 steps of all applications."""
+    if cfg is None:
+        cfg = IrConfig(tolerance=1e-4, max_outer=50, inner_tolerance=1e-1,
+                       restart=0, max_inner=100000,
+                       inner_method=IR_CG_RELIABLE)
+    c = _ir_struct(cfg)
+    h = C.c_void_p()
+    call("spk_precond_create_inner_ir", A.h, _h(M_inner), C.byref(c),
+         C.byref(h))
+    return InnerCg(h, "inner_ir", A.nrows, owner=(A, M_inner))
 
 
 FALLBACK_AUTO, FALLBACK_JACOBI_PCG, FALLBACK_ILU_GMRES, FALLBACK_NONE = 0, 1, 2, 3

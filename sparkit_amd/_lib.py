@@ -162,6 +162,8 @@ _SIGS = {
     "spk_precond_create_inner_cg": ([vp, vp, i64, dbl, i32, C.POINTER(vp)],
                                     None),
     "spk_precond_inner_cg_iterations": ([vp, P_i64], None),
+    "spk_precond_create_inner_ir": ([vp, vp, C.POINTER(IrConfig), C.POINTER(vp)],
+                                    None),
     "spk_learned_solve": ([vp, vp, vp, vp, vp, vp], None),
     "spk_axpby": ([i32, i64, dbl, vp, dbl, vp], None),
     "spk_convert": ([i64, i32, vp, i32, vp], None),

@@ -163,13 +163,13 @@ void cg_solve(spk_csr_s* A, spk_precond_s* ML, spk_precond_s* MR,
               spk_summary* out, double* history) {
   SPK_REQUIRE(cfg->restart > 0, SPK_INVALID_ARGUMENT,
               "restart_iterations must be > 0 (the reference divides by it)");
-  // an inner-PCG preconditioner runs this very solver (its work vectors):
-  // it belongs in FGMRES's flexible slot, not in CG's
-  SPK_REQUIRE((!ML || ML->kind != spk_precond_s::kInnerCg) &&
-                  (!MR || MR->kind != spk_precond_s::kInnerCg),
+  // an inner-solve preconditioner runs a solve of its own (an inner PCG
+  // this very solver and its work vectors): it belongs in FGMRES's flexible
+  // slot, not in CG's
+  SPK_REQUIRE((!ML || !ML->runs_solve()) && (!MR || !MR->runs_solve()),
               SPK_INVALID_ARGUMENT,
-              "conjugate_gradient: an inner-PCG preconditioner is not a fixed "
-              "operator (use it in fgmres)");
+              "conjugate_gradient: an inner-PCG or inner-solve preconditioner "
+              "is not a fixed operator (use it in fgmres)");
   const int64_t n = A->nrows;
   Workspace& ws = workspace();
   const size_t vb = (size_t)(n > 0 ? n : 1) * sizeof(T);

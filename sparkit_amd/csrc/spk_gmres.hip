@@ -269,9 +269,10 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
       }, stop);
     }
     const int64_t m_eff = std::min(m, cfg->max_iterations - iters);
-    // a preconditioner that runs a solve of its own (inner PCG) costs far
-    // more than a state read: stop the cycle as soon as it has converged
-    const bool check_each = !right_id && right->kind == spk_precond_s::kInnerCg;
+    // a preconditioner that runs a solve of its own (inner PCG, inner
+    // mixed-precision solve) costs far more than a state read: stop the
+    // cycle as soon as it has converged
+    const bool check_each = !right_id && right->runs_solve();
     for (int64_t j = 0; j < m_eff; ++j) {
       if (check_each && j > 0 && st.download().stop) break;
       T* vj = V + (size_t)j * nn;

@@ -487,6 +487,9 @@ extern "C" spk_status spk_ir_solve(spk_csr A, spk_precond M,
     SPK_REQUIRE(cfg->inner_method >= SPK_IR_GMRES &&
                     cfg->inner_method <= SPK_IR_CG_RELIABLE,
                 SPK_INVALID_ARGUMENT, "unknown inner method");
+    // an inner mixed-precision solve would reuse this solve's work vectors
+    SPK_REQUIRE(!M || M->kind != spk_precond_s::kInnerIr, SPK_INVALID_ARGUMENT,
+                "ir_solve: an inner-solve preconditioner cannot be used here");
     ir_solve(A, M, cfg, d_b, d_x, out, history);
     check_preconds(M, nullptr);
   });

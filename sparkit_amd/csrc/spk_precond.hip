@@ -621,6 +621,9 @@ void spk_precond_s::apply(const T* in, T* out) {
     case kInnerCg:
       inner_cg_apply(in, out);
       break;
+    case kInnerIr:
+      inner_ir_apply(in, out);
+      break;
   }
 }
 
@@ -670,6 +673,44 @@ void spk_precond_s::inner_cg_apply(const T* in, T* out) {
     inner_cg_run<float, T>(*this, in, out, ib32, iz32);
   else
     inner_cg_run<double, T>(*this, in, out, ib64, iz64);
+}
+
+// z = spk_ir_solve(inner_A z = r) from z = 0 with inner_M and inner_ir (the
+// solve's right-hand side and result are FP64: converted at the boundary
+// when the caller's vectors are FP32)
+template <typename T>
+void spk_precond_s::inner_ir_apply(const T* in, T* out) {
+  const double* b;
+  double* z;
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  if constexpr (std::is_same<T, double>::value) {
+    b = in;
+    z = out;
+    if (in == out) {  // the solve reads b while it writes z
+      ib64.reserve(m);
+      SPK_HIP_TRY(hipMemcpyAsync(ib64.p, in, (size_t)n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream()));
+      b = ib64.p;
+    }
+  } else {
+    ib64.reserve(m);
+    iz64.reserve(m);
+    double* bp = ib64.p;
+    launch_map(n, [=] __device__(int64_t i) { bp[i] = (double)in[i]; });
+    b = bp;
+    z = iz64.p;
+  }
+  launch_map(n, [=] __device__(int64_t i) { z[i] = 0.0; });
+  spk_ir_summary sum{};
+  const spk_status st = spk_ir_solve(inner_A, inner_M, &inner_ir, b, z, &sum,
+                                     nullptr);
+  if (st != SPK_OK && st != SPK_BREAKDOWN)
+    throw Error{st, std::string("inner mixed-precision solve: ") + spk_last_error()};
+  inner_total += sum.inner_iterations;
+  if constexpr (!std::is_same<T, double>::value) {
+    const double* zp = z;
+    launch_map(n, [=] __device__(int64_t i) { out[i] = (T)zp[i]; });
+  }
 }
 
 template void spk_precond_s::apply<double>(const double*, double*);
@@ -790,8 +831,8 @@ spk_status spk_precond_create_inner_cg(spk_csr A, spk_precond M_inner,
                 "inner PCG requires a square matrix");
     SPK_REQUIRE(!M_inner || M_inner->n == A->nrows, SPK_INVALID_ARGUMENT,
                 "inner preconditioner size mismatch");
-    SPK_REQUIRE(!M_inner || M_inner->kind != spk_precond_s::kInnerCg,
-                SPK_INVALID_ARGUMENT, "inner PCG cannot nest another one");
+    SPK_REQUIRE(!M_inner || !M_inner->runs_solve(), SPK_INVALID_ARGUMENT,
+                "inner PCG cannot nest another inner solve");
     SPK_REQUIRE(iterations >= 1 && tolerance >= 0.0, SPK_INVALID_ARGUMENT,
                 "iterations >= 1, tolerance >= 0");
     SPK_REQUIRE(precision == SPK_F32 || precision == SPK_F64,
@@ -808,10 +849,42 @@ spk_status spk_precond_create_inner_cg(spk_csr A, spk_precond M_inner,
   });
 }
 
+spk_status spk_precond_create_inner_ir(spk_csr A, spk_precond M_inner,
+                                       const spk_ir_config* cfg,
+                                       spk_precond* out) {
+  return guarded([&] {
+    SPK_REQUIRE(A && cfg && out, SPK_INVALID_ARGUMENT, "null argument");
+    SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
+                "inner solve requires a square matrix");
+    SPK_REQUIRE(!M_inner || M_inner->n == A->nrows, SPK_INVALID_ARGUMENT,
+                "inner preconditioner size mismatch");
+    SPK_REQUIRE(!M_inner || !M_inner->runs_solve(), SPK_INVALID_ARGUMENT,
+                "an inner solve cannot nest another inner solve");
+    // the (F)GMRES this preconditioner serves keeps its basis in the GMRES
+    // work vectors, which an inner (F)GMRES would reuse
+    SPK_REQUIRE(cfg->inner_method == SPK_IR_CG ||
+                    cfg->inner_method == SPK_IR_CG_RELIABLE ||
+                    cfg->inner_method == SPK_IR_BICGSTAB,
+                SPK_INVALID_ARGUMENT,
+                "inner solve: inner_method must be SPK_IR_CG, "
+                "SPK_IR_CG_RELIABLE or SPK_IR_BICGSTAB");
+    SPK_REQUIRE(cfg->tolerance > 0.0 && cfg->max_outer >= 1 && cfg->max_inner >= 1,
+                SPK_INVALID_ARGUMENT,
+                "inner solve: tolerance > 0, max_outer >= 1, max_inner >= 1");
+    auto* P = new spk_precond_s();
+    P->kind = spk_precond_s::kInnerIr;
+    P->n = A->nrows;
+    P->inner_A = A;
+    P->inner_M = M_inner;
+    P->inner_ir = *cfg;
+    *out = P;
+  });
+}
+
 spk_status spk_precond_inner_cg_iterations(spk_precond M, int64_t* total) {
   return guarded([&] {
-    SPK_REQUIRE(M && total && M->kind == spk_precond_s::kInnerCg,
-                SPK_INVALID_ARGUMENT, "not an inner PCG preconditioner");
+    SPK_REQUIRE(M && total && M->runs_solve(), SPK_INVALID_ARGUMENT,
+                "not an inner-solve preconditioner");
     *total = M->inner_total;
   });
 }

@@ -17,7 +17,7 @@
 #include "spk_csr.hpp"
 
 struct spk_precond_s {
-  enum Kind { kIdentity, kJacobi, kIlu, kSpmv, kInnerCg } kind = kIdentity;
+  enum Kind { kIdentity, kJacobi, kIlu, kSpmv, kInnerCg, kInnerIr } kind = kIdentity;
   int64_t n = 0;
   // Jacobi
   spk::DevBuf<double> inv_d64;
@@ -65,9 +65,15 @@ struct spk_precond_s {
   int64_t inner_iters = 0;
   double inner_tol = 0.0;
   int inner_precision = 1;  // SPK_F32
-  int64_t inner_total = 0;  // PCG steps run by all applications so far
+  int64_t inner_total = 0;  // inner steps run by all applications so far
   spk::DevBuf<float> ib32, iz32;
   spk::DevBuf<double> ib64, iz64;
+  // Inner mixed-precision solve (kInnerIr, also for FGMRES's flexible
+  // slot): z = the result of spk_ir_solve on inner_A z = r from z = 0 with
+  // inner_M and the configuration inner_ir.
+  spk_ir_config inner_ir{};
+  // a preconditioner that runs a solve of its own: not a fixed operator
+  bool runs_solve() const { return kind == kInnerCg || kind == kInnerIr; }
 
   ~spk_precond_s();
   const float* inv_diag32();
@@ -94,6 +100,8 @@ struct spk_precond_s {
   void jacobi_apply(const T* in, T* out);
   template <typename T>
   void inner_cg_apply(const T* in, T* out);
+  template <typename T>
+  void inner_ir_apply(const T* in, T* out);
 };
 
 template <>

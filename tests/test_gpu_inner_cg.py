@@ -66,3 +66,72 @@ def test_inner_pcg_rejected_where_a_fixed_operator_is_needed(spk):
         spk.inner_cg(A, P, iterations=5)  # no nesting
     with pytest.raises(spk.InvalidArgument):
         spk.inner_cg(A, None, iterations=0)
+
+
+def _ir_cfg(spk, tol=1e-6):
+    return spk.IrConfig(tolerance=tol, max_outer=30, inner_tolerance=0.1,
+                        restart=0, max_inner=20000,
+                        inner_method=spk.IR_CG_RELIABLE)
+
+
+def test_inner_ir_application_is_an_ir_solve(spk):
+    """spk_precond_create_inner_ir: one application is spk_ir_solve from
+    zero (FP32 PCG with FP64 reliable updates, the learned inverse's
+    symmetric part inside), bit for bit; in place; and for an FP32 caller
+    the FP64 result rounded."""
+    A = spk.aniso_q1_2d(96, 11, 1.5)
+    M = _learned_sym(spk, A)
+    cfg = _ir_cfg(spk)
+    P = spk.inner_ir(A, M, cfg)
+    r = spk.uniform_pm1(5, A.nrows)
+    z = spk.DeviceVector.zeros(A.nrows)
+    P.apply(r, z)
+    zz = spk.DeviceVector.zeros(A.nrows)
+    s = spk.ir_solve(r, zz, cfg, A, M)
+    assert s.converged and s.relres < 1e-6
+    assert P.inner_iterations == s.inner_iterations
+    assert np.array_equal(z.numpy(), zz.numpy())
+    P.apply(r, r)  # r is read before z is written
+    assert np.array_equal(r.numpy(), z.numpy())
+    r2 = spk.uniform_pm1(5, A.nrows)
+    r32 = r2.astype(np.float32)
+    z32 = spk.DeviceVector.zeros(A.nrows, np.float32)
+    P.apply(r32, z32)
+    b64 = spk.DeviceVector.from_numpy(r32.numpy().astype(np.float64))
+    z64 = spk.DeviceVector.zeros(A.nrows)
+    spk.ir_solve(b64, z64, cfg, A, M)
+    assert np.array_equal(z32.numpy(), z64.numpy().astype(np.float32))
+
+
+def test_fgmres_with_inner_ir_reaches_target(spk):
+    A = spk.aniso_q1_2d(128, 4242, 1.5)
+    b = spk.uniform_pm1(42, A.nrows)
+    M = _learned_sym(spk, A)
+    P = spk.inner_ir(A, M, _ir_cfg(spk, 1e-4))
+    x = spk.DeviceVector.zeros(A.nrows)
+    cfg = spk.FgmresConfig(1e-8, 30, 500, False, spk.F64, spk.ORTHO_CGS2)
+    s = spk.fgmres(b, x, cfg, A, P)
+    assert s.converged and spk.true_relres(A, b, x) < 1e-8
+    # each application cuts its residual 1e4-fold: a handful of outer steps
+    assert 1 <= s.computed_iterations <= 4
+    assert P.inner_iterations > 0
+
+
+def test_inner_ir_rejections(spk):
+    A = spk.poisson_2d(20, 20, 0.5)
+    J = spk.jacobi(A)
+    P = spk.inner_ir(A, J, _ir_cfg(spk))
+    b = spk.uniform_pm1(1, A.nrows)
+    x = spk.DeviceVector.zeros(A.nrows)
+    with pytest.raises(spk.InvalidArgument, match="inner-solve"):
+        spk.conjugate_gradient(b, x, spk.CGConfig(1e-8, 100, 100), A, P)
+    with pytest.raises(spk.InvalidArgument, match="inner-solve"):
+        spk.ir_solve(b, x, _ir_cfg(spk), A, P)
+    bad = _ir_cfg(spk)
+    bad.inner_method = spk.IR_GMRES  # would share the outer GMRES's vectors
+    with pytest.raises(spk.InvalidArgument, match="inner_method"):
+        spk.inner_ir(A, J, bad)
+    with pytest.raises(spk.InvalidArgument, match="nest"):
+        spk.inner_ir(A, P, _ir_cfg(spk))
+    with pytest.raises(spk.InvalidArgument, match="nest"):
+        spk.inner_cg(A, P, iterations=5)
