@@ -478,8 +478,16 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                          inner_method=spk.IR_CG_RELIABLE)
     run("learned_sym_relcg_fp32_update_every_10x",
         lambda x: spk.ir_solve(b, x, rcfg1, A, Ms))
+    # the same inverse with FP16 values in its FP32 SpMVs
+    # (spk_csr_set_half_values): half of its value bytes per step
+    Msym16 = model.predict(A).symmetric_part()
+    Msym16.set_half_values(True)
+    Ms16 = spk.spai(Msym16)
+    run("learned_sym_relcg_fp32_update_every_10x_m_fp16",
+        lambda x: spk.ir_solve(b, x, rcfg1, A, Ms16))
     for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32",
-              "learned_sym_relcg_fp32_update_every_10x"):
+              "learned_sym_relcg_fp32_update_every_10x",
+              "learned_sym_relcg_fp32_update_every_10x_m_fp16"):
         res[f"speedup_{k}_vs_jacobi_pcg"] = speedup(res, "jacobi_pcg_fp64", k)
     if pcg_only:  # the PCG comparisons only (no ILU(0)-PCG, no FGMRES)
         return res
@@ -536,11 +544,12 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
         icfg = spk.IrConfig(tolerance=tol, max_outer=200, inner_tolerance=1e-1,
                             restart=0, max_inner=50000,
                             inner_method=spk.IR_CG_RELIABLE)
-        P = spk.inner_ir(A, Ms, icfg)
-        name = f"fgmres30_inner_relcg_to_{tol:g}_learned_fp32"
-        run(name, lambda x: spk.fgmres(b, x, fg, A, P))
-        inner[name] = P
-        res[name]["inner_iterations_per_solve"] = P.inner_iterations // 2
+        for tag, Mi in (("learned", Ms), ("learned_m_fp16", Ms16)):
+            P = spk.inner_ir(A, Mi, icfg)
+            name = f"fgmres30_inner_relcg_to_{tol:g}_{tag}_fp32"
+            run(name, lambda x: spk.fgmres(b, x, fg, A, P))
+            inner[name] = P
+            res[name]["inner_iterations_per_solve"] = P.inner_iterations // 2
     guarded("guarded_fgmres30_learned", Ml, False)  # FGMRES above: warm
     poisoned = spk.SpaiModel(1)
     pp = model.params

@@ -210,9 +210,18 @@ spk_status spk_csr_get_panels(spk_csr A, int64_t* panels);
  * each x window in LDS, so every gather is an LDS read.  A row's sum runs
  * in column order with separately rounded products and sums: the
  * reference's storage-order sum (sparse_blas.hpp:26-31) when columns
- * ascend within rows (required: SPK_INVALID_ARGUMENT otherwise).  Replaces
- * column panels; the values are copied. */
+ * ascend within rows (required: SPK_INVALID_ARGUMENT otherwise).  Rows
+ * longer than max(256, 8 * windows) entries run in a store of their own
+ * by the row kernel, which splits a long row over lanes.  Replaces column
+ * panels; the values are copied. */
 spk_status spk_csr_set_window_tiles(spk_csr A, int64_t slots, int64_t window);
+/* No reference counterpart: FP32 SpMVs of A (plain layout) read its FP32
+ * values rounded to FP16 (round to nearest even), half the value bytes -- for a
+ * preconditioner store such as an approximate inverse applied in an FP32
+ * solve, whose FP64 true residual absorbs the rounding.  FP64 SpMVs are
+ * unaffected.  The FP16 copy is made here; SPK_INVALID_ARGUMENT if a finite
+ * value overflows FP16.  enable = 0 returns to the FP32 values. */
+spk_status spk_csr_set_half_values(spk_csr A, int enable);
 /* a layout by its code (as spk_csr_get_panels reports it) */
 spk_status spk_csr_set_layout(spk_csr A, int64_t code);
 /* times one FP64 SpMV for 1, 4, 8, 12, 16, 24, 32 and 64 panels (those

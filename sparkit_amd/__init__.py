@@ -286,6 +286,11 @@ class CsrMatrix:
         """A layout by its code (panels, or a negative window-tile code)."""
         call("spk_csr_set_layout", self.h, int(code))
 
+    def set_half_values(self, enable: bool = True):
+        """FP32 SpMVs read the values rounded to FP16
+        (spk_csr_set_half_values): for a preconditioner store."""
+        call("spk_csr_set_half_values", self.h, 1 if enable else 0)
+
     def set_panels(self, panels: int):
         call("spk_csr_set_panels", self.h, int(panels))
 

@@ -155,6 +155,7 @@ _SIGS = {
     "spk_csr_permute": ([vp, vp, i32, C.POINTER(vp)], None),
     "spk_csr_set_panels": ([vp, i64], None),
     "spk_csr_set_window_tiles": ([vp, i64, i64], None),
+    "spk_csr_set_half_values": ([vp, i32], None),
     "spk_csr_set_layout": ([vp, i64], None),
     "spk_csr_get_panels": ([vp, P_i64], None),
     "spk_csr_tune_layout": ([vp, P_i64, vp, vp, P_i64], None),

@@ -480,6 +480,9 @@ void require_ready(spk_csr_s& S, bool fp32) {
     return;
   }
   SPK_REQUIRE(!fp32 || S.has32, SPK_INVALID_ARGUMENT, msg32);
+  SPK_REQUIRE(!fp32 || !S.half32 || !S.panels.empty() || S.has16,
+              SPK_INVALID_ARGUMENT,
+              "no-allocation solve: a store's FP16 copy is not made yet");
   if (!S.panels.empty()) {
     SPK_REQUIRE(fp32 ? S.pacc32.n >= (size_t)S.nrows : S.pacc64.n >= (size_t)S.nrows,
                 SPK_INVALID_ARGUMENT, msgp);

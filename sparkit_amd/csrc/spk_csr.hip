@@ -52,6 +52,14 @@ __global__ void f64_to_f32_kernel(const double* in, float* out, int64_t n) {
     out[k] = (float)in[k];
 }
 
+// the FP32 value (as FP32 SpMVs read it) rounded to FP16, round to nearest
+// even at each step
+__global__ void f64_to_f16_kernel(const double* in, _Float16* out, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = (_Float16)(float)in[k];
+}
+
 // extract_diagonal: one thread per row, binary search (unary.hpp:29-37)
 __global__ void diag_kernel(const int32_t* rp, const int32_t* ci,
                             const double* v, int64_t len, double* d) {
@@ -63,6 +71,14 @@ __global__ void diag_kernel(const int32_t* rp, const int32_t* ci,
     if (ci[mid] < i) lo = mid + 1; else hi = mid;
   }
   d[i] = (lo < e && ci[lo] == i) ? v[lo] : 0.0;
+}
+
+// flags a finite FP64 value whose FP16 rounding is not finite
+__global__ void check_f16_kernel(const double* v, const _Float16* h, int64_t n,
+                                 int* bad) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    if (isfinite(v[k]) && !isfinite((float)h[k])) atomicOr(bad, 1);
 }
 
 int grid_for(int64_t n, int block = 256) {
@@ -523,6 +539,20 @@ const float* spk_csr_s::vals32() {
   return v32.p;
 }
 
+const _Float16* spk_csr_s::vals16() {
+  if (!has16) {
+    v16.alloc((size_t)nnz + spk::kPad);
+    hipStream_t s = spk::stream();
+    SPK_HIP_TRY(hipMemsetAsync(v16.p + nnz, 0, spk::kPad * sizeof(_Float16), s));
+    if (nnz > 0)
+      hipLaunchKernelGGL(spk::f64_to_f16_kernel, dim3(spk::grid_for(nnz)),
+                         dim3(256), 0, s, v64.p, v16.p, nnz);
+    SPK_HIP_TRY(hipGetLastError());
+    has16 = true;
+  }
+  return v16.p;
+}
+
 using namespace spk;
 
 extern "C" {
@@ -705,6 +735,31 @@ spk_status spk_csr_get_panels(spk_csr A, int64_t* panels) {
       *panels = -((int64_t)A->xt_slots * 65536 + A->xt_window);
     else
       *panels = A->panels.empty() ? 1 : (int64_t)A->panels.size();
+  });
+}
+
+spk_status spk_csr_set_half_values(spk_csr A, int enable) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    if (enable) {
+      // the FP16 copy is made here; a value beyond FP16's range becomes
+      // infinite there, which would make every FP32 product with it inf
+      (void)A->vals16();
+      DevBuf<int> bad(1);
+      hipStream_t s = stream();
+      SPK_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+      if (A->nnz > 0)
+        hipLaunchKernelGGL(check_f16_kernel, dim3(grid_for(A->nnz)), dim3(256), 0, s,
+                           (const double*)A->v64.p, (const _Float16*)A->v16.p,
+                           A->nnz, bad.p);
+      SPK_HIP_TRY(hipGetLastError());
+      int hbad = 0;
+      SPK_HIP_TRY(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      SPK_REQUIRE(!hbad, SPK_INVALID_ARGUMENT,
+                  "a finite value is out of FP16's range");
+    }
+    A->half32 = enable != 0;
   });
 }
 

@@ -58,6 +58,13 @@ struct spk_csr_s {
   spk::DevBuf<double> v64;
   spk::DevBuf<float> v32;
   bool has32 = false;
+  // FP16 values (half32): FP32 SpMVs of the plain layout read the values
+  // rounded to FP16 -- half the value bytes, for a preconditioner such as
+  // an approximate inverse, whose rounding the FP64 true residual of the
+  // solve it serves absorbs.  FP64 SpMVs keep v64.
+  spk::DevBuf<_Float16> v16;
+  bool has16 = false;
+  bool half32 = false;
   int nblocks = 0, nlong = 0;
   std::vector<int32_t> h_rp;  // host copy of the row offsets
   // Column panels (empty: none).  Panel p holds the entries with columns in
@@ -122,8 +129,12 @@ struct spk_csr_s {
   // allocates rp, ci and v64 (with the zeroed kPad tail)
   void allocate(int64_t nrows_, int64_t ncols_, int64_t nnz_);
   const float* vals32();
+  const _Float16* vals16();
   template <typename T>
   spk::CsrView<T> view();
+  spk::CsrView<_Float16> view16() {
+    return {rp.p, ci.p, vals16(), plan.p, nblocks, rows.p};
+  }
   template <typename T>
   T* panel_acc();
 };
@@ -238,14 +249,19 @@ template <>
 struct Pair<float> {
   typedef float type __attribute__((ext_vector_type(2)));
 };
+template <>
+struct Pair<_Float16> {
+  typedef _Float16 type __attribute__((ext_vector_type(2)));
+};
 typedef int IdxPair __attribute__((ext_vector_type(2)));
 
 // init (may be null): each row's sum starts from init[row] (column panels;
 // init may be the epilogue's own output).  Stored row r is matrix row
-// A.rows[r] when the store is compressed.
-template <typename T, typename Epi>
+// A.rows[r] when the store is compressed.  VT: the stored value type (T, or
+// FP16 values widened to T on load).
+template <typename T, typename Epi, typename VT = T>
 __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
-    CsrView<T> A, const T* __restrict__ x, Epi epi, double* partials,
+    CsrView<VT> A, const T* __restrict__ x, Epi epi, double* partials,
     const T* init) {
   __shared__ T prod[kNnzBlock + 2];
   __shared__ double red[kSpmvBlock / 64];
@@ -259,7 +275,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     // long row: the block holds exactly one row
     T s = T(0);
     for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
-      s += __builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
+      s += (T)__builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
     s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
     if (tid == 0) {
       const int ri = A.rows ? A.rows[r0] : r0;
@@ -274,7 +290,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     // load of the block and the row bounds are issued before the gathers
     const int k0 = base & ~1, off = base - k0, span = cnt + off;
     constexpr int U = (kNnzBlock + 2) / (2 * kSpmvBlock) + 1;
-    typedef typename Pair<T>::type TP;
+    typedef typename Pair<VT>::type TP;
     IdxPair c[U];
     TP a[U];
 #pragma unroll
@@ -285,7 +301,7 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
         a[u] = __builtin_nontemporal_load((const TP*)(A.v + k0 + k));
       } else {
         c[u] = IdxPair{0, 0};
-        a[u] = TP{T(0), T(0)};
+        a[u] = TP{VT(0), VT(0)};
       }
     }
     const int g = tid / L, lane = tid % L;
@@ -305,8 +321,8 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = 2 * tid + 2 * u * kSpmvBlock;
-      if (k < span) prod[k] = a[u].x * xv0[u];
-      if (k + 1 < span) prod[k + 1] = a[u].y * xv1[u];
+      if (k < span) prod[k] = (T)a[u].x * xv0[u];
+      if (k + 1 < span) prod[k + 1] = (T)a[u].y * xv1[u];
     }
     __syncthreads();
     if (L == 1) {
@@ -498,18 +514,31 @@ void launch_xtiles(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
   }
 }
 
+// the row kernel over one store: FP32 SpMVs of a store with half32 set
+// read its FP16 values
+template <typename T, typename Epi>
+void launch_row_kernel(spk_csr_s& A, const T* x, const Epi& epi, double* parts,
+                       const T* init, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (A.half32) {
+      hipLaunchKernelGGL((spmv_kernel<float, Epi, _Float16>), dim3(A.nblocks),
+                         dim3(kSpmvBlock), 0, s, A.view16(), x, epi, parts, init);
+      SPK_HIP_TRY(hipGetLastError());
+      return;
+    }
+  }
+  hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock), 0,
+                     s, A.view<T>(), x, epi, parts, init);
+  SPK_HIP_TRY(hipGetLastError());
+}
+
 template <typename T, typename Epi>
 void launch_spmv_one(spk_csr_s& A, const T* x, const Epi& epi, const T* init,
                      hipStream_t s) {
   const int nparts = A.xt_groups ? xtile_partials_count(A) : A.nblocks;
   double* parts = Epi::kReduce ? reduce_partials(nparts) : nullptr;
-  if (A.xt_groups) {
-    launch_xtiles(A, x, epi, parts, s);
-  } else {
-    hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(A.nblocks), dim3(kSpmvBlock),
-                       0, s, A.view<T>(), x, epi, parts, init);
-    SPK_HIP_TRY(hipGetLastError());
-  }
+  if (A.xt_groups) launch_xtiles(A, x, epi, parts, s);
+  else launch_row_kernel(A, x, epi, parts, init, s);
   if constexpr (Epi::kReduce) {
     hipLaunchKernelGGL((finish_kernel<1, EpiFinalize<Epi>>), dim3(1),
                        dim3(kFinishBlock), 0, s, (const double*)parts,
@@ -553,9 +582,7 @@ int launch_spmv_partials(spk_csr_s& A, const T* x, const Epi& epi, double* parts
   spk_csr_s& last = A.panels.empty() ? A : *A.panels.back();
   const T* init = nullptr;
   if (!A.panels.empty()) init = launch_panels(A, x, epi.done, s);
-  hipLaunchKernelGGL((spmv_kernel<T, Epi>), dim3(last.nblocks), dim3(kSpmvBlock),
-                     0, s, last.view<T>(), x, epi, parts, init);
-  SPK_HIP_TRY(hipGetLastError());
+  launch_row_kernel(last, x, epi, parts, init, s);
   return last.nblocks;
 }
 
