@@ -385,6 +385,25 @@ def mixed_precision_section(spk, grid=128, seed=42):
             f"{s64.computed_iterations} it {t64:.1f} ms relres {r64:.2e}; IR "
             f"{sir.outer_iterations} outer {sir.inner_iterations} inner "
             f"{tir:.1f} ms relres {rir:.2e}")
+    # the fastest form with the strict factors' values in FP16 for its FP32
+    # passes (spk_ilu0_set_half_factors; the FP64 refinement is unchanged)
+    F.jacobi_sweeps = 3
+    F.half_factors = True
+    xh = spk.DeviceVector.zeros(n)
+    icfg = spk.IrConfig(tolerance=1e-10, max_outer=20, inner_tolerance=1e-4,
+                        restart=50, max_inner=2000, inner_method=spk.IR_GMRES,
+                        orthogonalization=spk.ORTHO_CGS2)
+    spk.ir_solve(b, spk.DeviceVector.zeros(n),
+                 spk.IrConfig(1e-10, 1, 1e-2, 50, 50, spk.IR_GMRES,
+                              spk.ORTHO_CGS2), A, F)  # warm-up
+    sh, th = _timed(lambda: spk.ir_solve(b, xh, icfg, A, F))
+    res["ir_fp32_gmres50_ilu0jac3_fp16factors_cgs2"] = {
+        "outer_iterations": sh.outer_iterations,
+        "inner_iterations": sh.inner_iterations, "converged": sh.converged,
+        "ms": round(th, 2), "true_relres": spk.true_relres(A, b, xh)}
+    log(f"mixed precision (ilu0jac3, FP16 factors, cgs2): IR "
+        f"{sh.outer_iterations} outer {sh.inner_iterations} inner {th:.1f} ms")
+    F.half_factors = False
     F.jacobi_sweeps = 0
     # against the reference's method (FP64 GMRES(50), exact ILU(0) apply)
     base = min(res["fp64_gmres50_ilu0_cgs2"]["ms"],

@@ -282,6 +282,12 @@ spk_status spk_ilu0_get_schedule(spk_precond M, int* schedule);
  * instead of two latency-bound sweeps.  spk_ilu0_forward/backward stay
  * exact.  k in [0, 1024]. */
 spk_status spk_ilu0_set_jacobi_sweeps(spk_precond M, int sweeps);
+/* No reference counterpart: the Jacobi-sweep apply's FP32 passes read the
+ * strict factors' FP32 values rounded to FP16 (spk_csr_set_half_values on
+ * L_s and U_s; U's diagonal stays FP32), for FP32 inner solves under an
+ * FP64 refinement.  FP64 applies and the exact sweeps are unaffected. */
+spk_status spk_ilu0_set_half_factors(spk_precond M, int enable);
+spk_status spk_ilu0_get_half_factors(spk_precond M, int* enabled);
 spk_status spk_ilu0_get_jacobi_sweeps(spk_precond M, int* sweeps);
 /* Identity slot (conjugate_gradient_test.cpp:45-47). */
 spk_status spk_precond_create_identity(int64_t n, spk_precond* out);

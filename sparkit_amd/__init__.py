@@ -424,6 +424,18 @@ class Ilu0(Preconditioner):
     def jacobi_sweeps(self, k: int):
         call("spk_ilu0_set_jacobi_sweeps", self.h, int(k))
 
+    @property
+    def half_factors(self) -> bool:
+        """The Jacobi-sweep apply's FP32 passes read the strict factors'
+        values in FP16 (spk_ilu0_set_half_factors)."""
+        v = C.c_int()
+        call("spk_ilu0_get_half_factors", self.h, C.byref(v))
+        return bool(v.value)
+
+    @half_factors.setter
+    def half_factors(self, enable: bool):
+        call("spk_ilu0_set_half_factors", self.h, 1 if enable else 0)
+
     def forward(self, b: DeviceVector, x: DeviceVector):
         call("spk_ilu0_forward", self.h, _code(b.dtype), b.ptr, x.ptr)
 

@@ -793,6 +793,28 @@ spk_status spk_ilu0_set_jacobi_sweeps(spk_precond M, int sweeps) {
   });
 }
 
+spk_status spk_ilu0_set_half_factors(spk_precond M, int enable) {
+  return guarded([&] {
+    SPK_REQUIRE(M && M->kind == spk_precond_s::kIlu, SPK_INVALID_ARGUMENT,
+                "not an ILU(0) preconditioner");
+    SPK_HIP_TRY(hipStreamSynchronize(stream()));
+    M->build_strict();
+    for (spk_csr_s* S : {M->Ls, M->Us}) {
+      const spk_status st = spk_csr_set_half_values(S, enable);
+      if (st != SPK_OK)
+        throw Error{st, std::string("ILU(0) strict factors: ") + spk_last_error()};
+    }
+  });
+}
+
+spk_status spk_ilu0_get_half_factors(spk_precond M, int* enabled) {
+  return guarded([&] {
+    SPK_REQUIRE(M && enabled && M->kind == spk_precond_s::kIlu,
+                SPK_INVALID_ARGUMENT, "not an ILU(0) preconditioner");
+    *enabled = M->Ls && M->Ls->half32 ? 1 : 0;
+  });
+}
+
 spk_status spk_ilu0_get_jacobi_sweeps(spk_precond M, int* sweeps) {
   return guarded([&] {
     SPK_REQUIRE(M && sweeps && M->kind == spk_precond_s::kIlu,

@@ -89,3 +89,35 @@ def test_learned_relcg_with_half_inverse(spk):
         assert s.converged and spk.true_relres(A, b, x) < 1e-8
         its.append(s.inner_iterations)
     assert its[1] <= 1.1 * its[0]
+
+
+def test_ilu_half_factors_in_fp32_refinement(spk):
+    """The Jacobi-swept ILU(0) apply with FP16 strict factors: the FP32
+    apply moves by about FP16's rounding, the FP64 apply not at all, and
+    FP32-inner GMRES(50) under FP64 refinement still reaches 1e-10."""
+    A = spk.convdiff_3d(40, 40, 40, 1.0, 1.35, 0.68, 0.34)
+    F = spk.ilu0(A)
+    F.jacobi_sweeps = 3
+    r = spk.uniform_pm1(3, A.nrows)
+    r32 = r.astype(np.float32)
+
+    def apply(v):
+        z = spk.DeviceVector.zeros(A.nrows, v.dtype)
+        F.apply(v, z)
+        return z.numpy()
+
+    z64, z32 = apply(r), apply(r32)
+    F.half_factors = True
+    assert F.half_factors
+    h32 = apply(r32)
+    assert np.array_equal(apply(r), z64)
+    d = np.linalg.norm(h32.astype(np.float64) - z32) / np.linalg.norm(z32)
+    assert 1e-6 < d < 1e-2
+    b = spk.uniform_pm1(42, A.nrows)
+    x = spk.DeviceVector.zeros(A.nrows)
+    cfg = spk.IrConfig(1e-10, 20, 1e-4, 50, 2000, spk.IR_GMRES, spk.ORTHO_CGS2)
+    s = spk.ir_solve(b, x, cfg, A, F)
+    assert s.converged and spk.true_relres(A, b, x) < 1e-10
+    F.half_factors = False
+    assert not F.half_factors
+    assert np.array_equal(apply(r32), z32)
