@@ -121,3 +121,10 @@ def test_ilu_half_factors_in_fp32_refinement(spk):
     F.half_factors = False
     assert not F.half_factors
     assert np.array_equal(apply(r32), z32)
+
+
+def test_half_values_empty_store(O, spk):
+    E = spk.CsrMatrix.from_oracle(O.make_matrix(3, 3, [], [], []))
+    E.set_half_values(True)
+    x = spk.DeviceVector.from_numpy(np.array([1., 2., 3.], np.float32))
+    assert spk.multiply(E, x).numpy().tolist() == [0., 0., 0.]

@@ -386,3 +386,41 @@ def test_window_tiles_in_solvers_and_layout_switches(O, spk):
     with pytest.raises(spk.InvalidArgument, match="ascending"):
         U.set_window_tiles(8, 256)
     assert U.panels == 1
+
+
+def test_window_tiles_edge_cases(O, spk):
+    """sparse_blas_test.cpp:56-95's empty, rectangular and identity cases
+    under window tiles (an empty store stays plain), and a store whose
+    every row is long: the tiles hold no entry and the long-row store sums
+    every row."""
+    E = spk.CsrMatrix.from_oracle(O.make_matrix(3, 3, [], [], []))
+    E.set_window_tiles(8, 256)
+    assert E.panels == 1
+    x = spk.DeviceVector.from_numpy(np.array([1., 2., 3.]))
+    assert spk.multiply(E, x).numpy().tolist() == [0., 0., 0.]
+    R = spk.CsrMatrix.from_oracle(O.make_matrix(2, 3, [0, 0, 1], [0, 2, 1],
+                                                [1., 2., 3.]))
+    R.set_window_tiles(8, 256)
+    assert spk.multiply(R, x).numpy().tolist() == [7., 6.]
+    I = spk.CsrMatrix.from_oracle(O.make_matrix(3, 3, [0, 1, 2], [0, 1, 2],
+                                                [1., 1., 1.]))
+    I.set_window_tiles(16, 8192)
+    assert spk.multiply(I, spk.DeviceVector.from_numpy(
+        np.array([4., 5., 6.]))).numpy().tolist() == [4., 5., 6.]
+    rng = np.random.default_rng(11)
+    n, nc = 40, 3000  # one window of 8192 columns: rows over 256 are long
+    cols = [np.sort(rng.choice(nc, size=400, replace=False)) for _ in range(n)]
+    rp = np.arange(n + 1) * 400
+    ci = np.concatenate(cols).astype(np.int64)
+    v = rng.uniform(-1, 1, len(ci))
+    L = spk.CsrMatrix.from_host(n, nc, rp, ci, v)
+    L.set_window_tiles(16, 8192)
+    xl = O.uniform_pm1(2, nc)
+    y = spk.multiply(L, spk.DeviceVector.from_numpy(xl)).numpy()
+    ref = O.spmv(O.Csr(n, nc, rp, ci, v), xl)
+    absrow = np.abs(O.Csr(n, nc, rp, ci, v).to_scipy()) @ np.abs(xl)
+    assert np.all(np.abs(y - ref) <= 1e-12 * absrow)
+    yd = spk.DeviceVector.zeros(n)
+    r = spk.DeviceVector.zeros(1)
+    with pytest.raises(spk.InvalidArgument):
+        L.spmv_dot(spk.DeviceVector.from_numpy(xl), yd, r)  # not square
