@@ -598,6 +598,45 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     return res
 
 
+def short_index_section(spk, reps=50):
+    """SpMV time with 16-bit column offsets (spk_csr_set_short_indices)
+    against the 32-bit indices, FP64 and FP32, on banded natural-order
+    stencil matrices (mean of reps calls after 5 warm-up calls; the results
+    are equal bit for bit)."""
+    def t(A, dt):
+        x = spk.DeviceVector.from_numpy(np.ones(A.ncols, dt))
+        y = spk.DeviceVector.zeros(A.nrows, dt)
+        for _ in range(5):
+            A.spmv(x, y)
+        spk.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            A.spmv(x, y)
+        spk.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e6
+    res = {}
+    for name, mk in (
+            ("2D anisotropic Q1 2048^2", lambda: spk.aniso_q1_2d(2048, 7, 1.5)),
+            ("3D 7-point Poisson 160^3", lambda: spk.poisson_3d(160, 160, 160)),
+            ("2D anisotropic Q1 1024^2", lambda: spk.aniso_q1_2d(1024, 4242, 1.5)),
+            ("3D upwind convection-diffusion 128^3",
+             lambda: spk.convdiff_3d(128, 128, 128, 1.0, 0.3, 0.15, 0.07))):
+        A = mk()
+        d = {"rows": A.nrows, "nnz": A.nnz}
+        for dt in (np.float64, np.float32):
+            a = t(A, dt)
+            A.set_short_indices(True)
+            b = t(A, dt)
+            A.set_short_indices(False)
+            k = np.dtype(dt).name
+            d[k] = {"int32_us": round(a, 1), "short_us": round(b, 1),
+                    "speedup": round(a / b, 3)}
+        res[name] = d
+        log(f"short indices: {name}: {d}")
+        del A
+    return res
+
+
 def krylov_section(spk, grid=128, seed=7):
     """The other Krylov solvers on the device, FP64 + Jacobi to a 1e-8
     relative residual: nonsymmetric 3D upwind convection-diffusion grid^3
@@ -916,7 +955,7 @@ def main():
                          "negative code -(slots * 65536 + window) selects "
                          "window tiles")
     ap.add_argument("--sections",
-                    default="p2d,mp,spai,batched,krylov,mmio,rcm",
+                    default="p2d,mp,spai,batched,krylov,mmio,rcm,short-idx",
                     help="extra measurements after the timed steps: "
                          "p2d (poisson_2d(256) PCG), mp (mixed precision), "
                          "spai (learned SPAI), spai-pcg (its PCG "
@@ -924,6 +963,7 @@ def main():
                          "batched (batched PCG), krylov (BiCGSTAB, CGS, "
                          "BiCG, QMR, CG, MINRES), mmio (Matrix Market "
                          "round trip), rcm (device reverse Cuthill-McKee), "
+                         "short-idx (SpMV with 16-bit column offsets), "
                          "spai-ceiling (learned vs Frobenius-optimal SPAI; "
                          "not in the default list); 'none' for neither")
     ap.add_argument("--spai-epochs", type=int, default=24)
@@ -1107,6 +1147,8 @@ def main():
         section("matrix_market", lambda: mmio_section(spk))
     if "rcm" in secs:
         section("rcm", lambda: rcm_section(spk))
+    if "short-idx" in secs:
+        section("short_indices", lambda: short_index_section(spk))
     if "spai-ceiling" in secs:
         section("spai_ceiling", lambda: spai_ceiling_section(
             spk, n_side=args.spai_side, train_count=args.spai_train,

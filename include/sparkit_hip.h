@@ -222,6 +222,15 @@ spk_status spk_csr_set_window_tiles(spk_csr A, int64_t slots, int64_t window);
  * unaffected.  The FP16 copy is made here; SPK_INVALID_ARGUMENT if a finite
  * value overflows FP16.  enable = 0 returns to the FP32 values. */
 spk_status spk_csr_set_half_values(spk_csr A, int enable);
+/* No reference counterpart: SpMVs of A's plain layout read each column
+ * index as a 16-bit offset from the first row of its row block (2 bytes per
+ * index instead of 4, in every precision; results unchanged bit for bit).
+ * For banded stores in natural order, such as stencil matrices: every
+ * column must lie within 32767 of its block's first row
+ * (SPK_INVALID_ARGUMENT otherwise, and the store keeps its 32-bit reads).
+ * Column panels and window tiles are unaffected.  enable = 0 returns to the
+ * 32-bit indices. */
+spk_status spk_csr_set_short_indices(spk_csr A, int enable);
 /* a layout by its code (as spk_csr_get_panels reports it) */
 spk_status spk_csr_set_layout(spk_csr A, int64_t code);
 /* times one FP64 SpMV for 1, 4, 8, 12, 16, 24, 32 and 64 panels (those

@@ -291,6 +291,11 @@ class CsrMatrix:
         (spk_csr_set_half_values): for a preconditioner store."""
         call("spk_csr_set_half_values", self.h, 1 if enable else 0)
 
+    def set_short_indices(self, enable: bool = True):
+        """The row kernel reads 16-bit column offsets from each row block's
+        first row (spk_csr_set_short_indices): banded stores only."""
+        call("spk_csr_set_short_indices", self.h, 1 if enable else 0)
+
     def set_panels(self, panels: int):
         call("spk_csr_set_panels", self.h, int(panels))
 

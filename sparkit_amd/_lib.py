@@ -156,6 +156,7 @@ _SIGS = {
     "spk_csr_set_panels": ([vp, i64], None),
     "spk_csr_set_window_tiles": ([vp, i64, i64], None),
     "spk_csr_set_half_values": ([vp, i32], None),
+    "spk_csr_set_short_indices": ([vp, i32], None),
     "spk_ilu0_set_half_factors": ([vp, i32], None),
     "spk_ilu0_get_half_factors": ([vp, C.POINTER(C.c_int)], None),
     "spk_csr_set_layout": ([vp, i64], None),

@@ -81,6 +81,19 @@ __global__ void check_f16_kernel(const double* v, const _Float16* h, int64_t n,
     if (isfinite(v[k]) && !isfinite((float)h[k])) atomicOr(bad, 1);
 }
 
+// short column indices: one workgroup per row block writes its entries'
+// columns as offsets from the block's first row, flagging any beyond int16
+__global__ void short_cols_kernel(const int2* __restrict__ plan,
+                                  const int32_t* __restrict__ ci, int16_t* c16,
+                                  int* bad) {
+  const int2 p0 = plan[blockIdx.x], p1 = plan[blockIdx.x + 1];
+  for (int k = p0.y + threadIdx.x; k < p1.y; k += blockDim.x) {
+    const int d = ci[k] - p0.x;
+    if (d < -32768 || d > 32767) atomicOr(bad, 1);
+    else c16[k] = (int16_t)d;
+  }
+}
+
 int grid_for(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 65536);
@@ -760,6 +773,34 @@ spk_status spk_csr_set_half_values(spk_csr A, int enable) {
                   "a finite value is out of FP16's range");
     }
     A->half32 = enable != 0;
+  });
+}
+
+spk_status spk_csr_set_short_indices(spk_csr A, int enable) {
+  return guarded([&] {
+    SPK_REQUIRE(A, SPK_INVALID_ARGUMENT, "null store");
+    if (!enable) {
+      A->short_idx = false;
+      return;
+    }
+    if (A->short_idx) return;
+    hipStream_t s = stream();
+    A->c16.alloc((size_t)A->nnz + kPad);
+    SPK_HIP_TRY(hipMemsetAsync(A->c16.p + A->nnz, 0, kPad * sizeof(int16_t), s));
+    DevBuf<int> bad(1);
+    SPK_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    if (A->nblocks > 0)
+      hipLaunchKernelGGL(short_cols_kernel, dim3(A->nblocks), dim3(256), 0, s,
+                         (const int2*)A->plan.p, (const int32_t*)A->ci.p, A->c16.p,
+                         bad.p);
+    SPK_HIP_TRY(hipGetLastError());
+    int hbad = 0;
+    SPK_HIP_TRY(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    if (hbad) A->c16.release();
+    SPK_REQUIRE(!hbad, SPK_INVALID_ARGUMENT,
+                "a column lies more than 32767 from its row block's first row");
+    A->short_idx = true;
   });
 }
 

@@ -7,6 +7,7 @@
 //   v64[nnz+kPad]    double  values           (Compressed_row_matrix::values)
 //   v32[nnz+kPad]    float   FP32 copy, made on first FP32 use
 //   plan[nb+1]       int2    {first row, first nonzero} of each SpMV row block
+//   c16[nnz+kPad]    int16   optional: column - first row of the entry's block
 // The kPad zeroed tail lets the kernel read index/value PAIRS from an even
 // start without leaving the allocation.
 //
@@ -47,6 +48,8 @@ struct CsrView {
   const int2* __restrict__ plan;  // {first row, first nonzero} per block
   int nblocks;
   const int32_t* __restrict__ rows;  // matrix row of each stored row (null: same)
+  // null, or short column indices: column = the block's first row + c16[k]
+  const int16_t* __restrict__ c16;
 };
 
 }  // namespace spk
@@ -65,6 +68,15 @@ struct spk_csr_s {
   spk::DevBuf<_Float16> v16;
   bool has16 = false;
   bool half32 = false;
+  // Short column indices (short_idx): c16[k] = column - the first row of
+  // entry k's row block, for stores whose every entry lies within 32767
+  // columns of its block's first row (banded matrices in natural order,
+  // such as stencils): the row kernel then streams 2 index bytes per entry
+  // instead of 4, in every precision.  Column panels and window tiles keep
+  // their own indices.
+  spk::DevBuf<int16_t> c16;
+  bool short_idx = false;
+  const int16_t* short_cols() const { return short_idx ? c16.p : nullptr; }
   int nblocks = 0, nlong = 0;
   std::vector<int32_t> h_rp;  // host copy of the row offsets
   // Column panels (empty: none).  Panel p holds the entries with columns in
@@ -133,7 +145,7 @@ struct spk_csr_s {
   template <typename T>
   spk::CsrView<T> view();
   spk::CsrView<_Float16> view16() {
-    return {rp.p, ci.p, vals16(), plan.p, nblocks, rows.p};
+    return {rp.p, ci.p, vals16(), plan.p, nblocks, rows.p, short_cols()};
   }
   template <typename T>
   T* panel_acc();
@@ -152,11 +164,11 @@ inline float* spk_csr_s::panel_acc<float>() {
 
 template <>
 inline spk::CsrView<double> spk_csr_s::view<double>() {
-  return {rp.p, ci.p, v64.p, plan.p, nblocks, rows.p};
+  return {rp.p, ci.p, v64.p, plan.p, nblocks, rows.p, short_cols()};
 }
 template <>
 inline spk::CsrView<float> spk_csr_s::view<float>() {
-  return {rp.p, ci.p, vals32(), plan.p, nblocks, rows.p};
+  return {rp.p, ci.p, vals32(), plan.p, nblocks, rows.p, short_cols()};
 }
 
 namespace spk {
@@ -255,31 +267,38 @@ struct Pair<_Float16> {
 };
 typedef int IdxPair __attribute__((ext_vector_type(2)));
 
-// init (may be null): each row's sum starts from init[row] (column panels;
-// init may be the epilogue's own output).  Stored row r is matrix row
-// A.rows[r] when the store is compressed.  VT: the stored value type (T, or
-// FP16 values widened to T on load).
-template <typename T, typename Epi, typename VT = T>
-__global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
-    CsrView<VT> A, const T* __restrict__ x, Epi epi, double* partials,
-    const T* init) {
-  __shared__ T prod[kNnzBlock + 2];
-  __shared__ double red[kSpmvBlock / 64];
-  if (epi.done && *epi.done) return;
+// The row kernel's work for one block: the products of its entries in
+// storage order and one sum per row.  x[j] is read as gx(j).  pre() runs
+// after the block's index and value loads and its row bounds are issued and
+// before the first gather (a hand-off kernel sums the previous kernel's
+// partials there, so their latency overlaps the loads); every thread of the
+// block calls it, and false ends the block.  row(ri, s, acc) gets each
+// row's sum; W: the number of block sums in acc.  init (may be null): each
+// row's sum starts from init[row] (column panels; init may be the
+// epilogue's own output).  Stored row r is matrix row A.rows[r] when the
+// store is compressed.  VT: the stored value type (T, or FP16 values
+// widened to T on load).
+template <int W, typename T, typename VT, typename Pre, typename Gx, typename Row>
+__device__ __forceinline__ void spmv_rows(const CsrView<VT>& A, const T* init,
+                                          Pre&& pre, const Gx& gx, const Row& row,
+                                          T* prod, double* red, double* acc) {
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int2 p0 = A.plan[b], p1 = A.plan[b + 1];
   const int r0 = p0.x, base = p0.y, nrows = p1.x - p0.x, cnt = p1.y - p0.y;
-  double acc = 0.0;
   if (cnt > kNnzBlock) {
     // long row: the block holds exactly one row
+    if (!pre()) return;
     T s = T(0);
-    for (int k = base + tid; k < base + cnt; k += kSpmvBlock)
-      s += (T)__builtin_nontemporal_load(A.v + k) * x[__builtin_nontemporal_load(A.ci + k)];
+    for (int k = base + tid; k < base + cnt; k += kSpmvBlock) {
+      const int j = A.c16 ? r0 + (int)__builtin_nontemporal_load(A.c16 + k)
+                          : __builtin_nontemporal_load(A.ci + k);
+      s += (T)__builtin_nontemporal_load(A.v + k) * gx(j);
+    }
     s = block_reduce_sum<T, kSpmvBlock>(s, reinterpret_cast<T*>(red));
     if (tid == 0) {
       const int ri = A.rows ? A.rows[r0] : r0;
-      acc = epi.row(ri, init ? init[ri] + s : s);
+      row(ri, init ? init[ri] + s : s, acc);
     }
   } else {
     // lanes per row: enough rows to occupy the block, about eight products
@@ -293,15 +312,35 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
     typedef typename Pair<VT>::type TP;
     IdxPair c[U];
     TP a[U];
+    if (A.c16) {
+      // short indices, two per 32-bit load; the entry before base that an
+      // odd base brings in belongs to the previous block (its offset is
+      // from that block's first row): its column is replaced by 0, and its
+      // product is never summed
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = 2 * tid + 2 * u * kSpmvBlock;
-      if (k < span) {
-        c[u] = __builtin_nontemporal_load((const IdxPair*)(A.ci + k0 + k));
-        a[u] = __builtin_nontemporal_load((const TP*)(A.v + k0 + k));
-      } else {
-        c[u] = IdxPair{0, 0};
-        a[u] = TP{VT(0), VT(0)};
+      for (int u = 0; u < U; ++u) {
+        const int k = 2 * tid + 2 * u * kSpmvBlock;
+        if (k < span) {
+          const int w = __builtin_nontemporal_load((const int*)(A.c16 + k0 + k));
+          c[u] = IdxPair{r0 + (int)(int16_t)(w & 0xffff), r0 + (w >> 16)};
+          a[u] = __builtin_nontemporal_load((const TP*)(A.v + k0 + k));
+        } else {
+          c[u] = IdxPair{0, 0};
+          a[u] = TP{VT(0), VT(0)};
+        }
+      }
+      if (2 * tid < off) c[0].x = 0;
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = 2 * tid + 2 * u * kSpmvBlock;
+        if (k < span) {
+          c[u] = __builtin_nontemporal_load((const IdxPair*)(A.ci + k0 + k));
+          a[u] = __builtin_nontemporal_load((const TP*)(A.v + k0 + k));
+        } else {
+          c[u] = IdxPair{0, 0};
+          a[u] = TP{VT(0), VT(0)};
+        }
       }
     }
     const int g = tid / L, lane = tid % L;
@@ -311,12 +350,13 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
       s1 = A.rp[r0 + g + 1] - base + off;
       if (A.rows) ri = A.rows[r0 + g];
     }
+    if (!pre()) return;
     T xv0[U], xv1[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = 2 * tid + 2 * u * kSpmvBlock;
-      xv0[u] = (k < span) ? x[c[u].x] : T(0);
-      xv1[u] = (k + 1 < span) ? x[c[u].y] : T(0);
+      xv0[u] = (k < span) ? gx(c[u].x) : T(0);
+      xv1[u] = (k + 1 < span) ? gx(c[u].y) : T(0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -330,20 +370,32 @@ __global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
       if (g < nrows) {
         T s = init ? init[ri] : T(0);
         for (int j = s0; j < s1; ++j) s += prod[j];
-        acc = epi.row(ri, s);
+        row(ri, s, acc);
       }
     } else {
       T s = T(0);
       if (g < nrows)
         for (int j = s0 + lane; j < s1; j += L) s += prod[j];
       for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if (g < nrows && lane == 0)
-        acc = epi.row(ri, init ? init[ri] + s : s);
+      if (g < nrows && lane == 0) row(ri, init ? init[ri] + s : s, acc);
     }
   }
+}
+
+template <typename T, typename Epi, typename VT = T>
+__global__ __launch_bounds__(kSpmvBlock) void spmv_kernel(
+    CsrView<VT> A, const T* __restrict__ x, Epi epi, double* partials,
+    const T* init) {
+  __shared__ T prod[kNnzBlock + 2];
+  __shared__ double red[kSpmvBlock / 64];
+  if (epi.done && *epi.done) return;
+  double acc = 0.0;
+  spmv_rows<1>(
+      A, init, [] { return true; }, [x](int j) { return x[j]; },
+      [&epi](int ri, T s, double* a) { a[0] = epi.row(ri, s); }, prod, red, &acc);
   if constexpr (Epi::kReduce) {
     const double t = block_sum_to_thread0<kSpmvBlock>(acc, red);
-    if (tid == 0) partials[b] = t;
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
   }
 }
 
