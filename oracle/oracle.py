@@ -15,7 +15,9 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SPK_ORACLE_LIB: another build of the same sources (the AddressSanitizer +
+# UndefinedBehaviorSanitizer one, oracle/_san/liboracle_san.so)
+LIB_PATH = os.environ.get("SPK_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

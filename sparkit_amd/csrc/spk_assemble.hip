@@ -231,6 +231,7 @@ using namespace spk;
 
 extern "C" spk_status spk_csr_symmetric_part(spk_csr M, spk_csr* out) {
   return guarded([&] {
+    TraceRange trace_("spk_csr_symmetric_part");
     SPK_REQUIRE(M && out, SPK_INVALID_ARGUMENT, "null argument");
     *out = symmetric_part(*M);
   });
@@ -238,6 +239,7 @@ extern "C" spk_status spk_csr_symmetric_part(spk_csr M, spk_csr* out) {
 
 extern "C" spk_status spk_csr_transpose(spk_csr A, spk_csr* out) {
   return guarded([&] {
+    TraceRange trace_("spk_csr_transpose");
     SPK_REQUIRE(A && out, SPK_INVALID_ARGUMENT, "null argument");
     *out = transpose(*A);
   });
@@ -247,6 +249,7 @@ extern "C" spk_status spk_csr_assemble(int64_t nrows, int64_t ncols, int64_t cou
                                        const int64_t* d_rows, const int64_t* d_cols,
                                        const double* d_vals, spk_csr* out) {
   return guarded([&] {
+    TraceRange trace_("spk_csr_assemble");
     SPK_REQUIRE(out, SPK_INVALID_ARGUMENT, "null output");
     *out = assemble(nrows, ncols, count, d_rows, d_cols, d_vals);
   });

@@ -920,6 +920,7 @@ spk_status spk_batch_pcg_solve(spk_batch B, const spk_batch_config* cfg,
                                const double* d_b, double* d_x, int use_graph,
                                spk_batch_summary* out) {
   return guarded([&] {
+    TraceRange trace_("spk_batch_pcg_solve");
     SPK_REQUIRE(B && d_b && d_x, SPK_INVALID_ARGUMENT, "null argument");
     check_config(cfg);
     check_tuning(B->tuning, SPK_F32);  // lds_rows is capped per precision
@@ -932,6 +933,7 @@ spk_status spk_batch_pcg_solve(spk_batch B, const spk_batch_config* cfg,
                            t.lds_rows, cfg->precision};
       auto it = B->graphs.find(key);
       if (it == B->graphs.end()) {
+        TraceRange capture_("batch: graph capture");
         hipGraph_t g = nullptr;
         SPK_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
@@ -948,6 +950,7 @@ spk_status spk_batch_pcg_solve(spk_batch B, const spk_batch_config* cfg,
         SPK_HIP_TRY(e);
         it = B->graphs.emplace(key, ex).first;
       }
+      TraceRange replay_("batch: graph replay");
       SPK_HIP_TRY(hipGraphLaunch(it->second, s));
     } else {
       B->enqueue(*cfg, d_b, d_x, t, B->order.p, (int)B->K, B->total);
@@ -997,6 +1000,7 @@ spk_status spk_batch_autotune(spk_batch B, const spk_batch_config* cfg,
                               spk_batch_tuning* chosen, spk_batch_tuning* tried,
                               float* tried_ms, int64_t* ntried) {
   return guarded([&] {
+    TraceRange trace_("spk_batch_autotune");
     SPK_REQUIRE(B && d_b && d_x && chosen, SPK_INVALID_ARGUMENT, "null argument");
     check_config(cfg);
     if (cfg->precision == SPK_F64) B->ensure_f64();

@@ -345,6 +345,7 @@ extern "C" spk_status spk_bicg_solve(spk_csr A, spk_csr At, spk_precond M_left,
                                      void* d_x, spk_summary* summary,
                                      double* history) {
   return guarded([&] {
+    TraceRange trace_("spk_bicg_solve");
     check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
     SPK_REQUIRE(At && summary, SPK_INVALID_ARGUMENT, "null argument");
     SPK_REQUIRE(At->nrows == A->ncols && At->ncols == A->nrows && At->nnz == A->nnz,

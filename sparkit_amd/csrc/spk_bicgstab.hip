@@ -424,6 +424,7 @@ extern "C" spk_status spk_bicgstab_solve(spk_csr A, spk_precond M_left,
                                          spk_summary* summary,
                                          double* history) {
   return guarded([&] {
+    TraceRange trace_("spk_bicgstab_solve");
     check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
     SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
     *summary = spk_summary{};

@@ -830,6 +830,7 @@ spk_status spk_csr_set_layout(spk_csr A, int64_t code) {
 spk_status spk_csr_tune_layout(spk_csr A, int64_t* chosen, int64_t* tried,
                                float* tried_us, int64_t* ntried) {
   return guarded([&] {
+    TraceRange trace_("spk_csr_tune_layout");
     SPK_REQUIRE(A && chosen, SPK_INVALID_ARGUMENT, "null argument");
     hipStream_t s = stream();
     DevBuf<double> x((size_t)std::max<int64_t>(A->ncols, 1));

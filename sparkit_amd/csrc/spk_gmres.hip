@@ -234,6 +234,7 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
 
   int64_t iters = 0;
   while (iters < cfg->max_iterations && !h.converged && !h.done) {
+    TraceRange cycle_("gmres restart cycle");
     // compute_initial_residual (gmres.hpp:211-241 / fgmres.hpp:206-224)
     if (is_identity(left)) {
       launch_spmv(*A, (const T*)x, EpiResidual<T>{b, V}, s);
@@ -403,6 +404,7 @@ spk_status spk_gmres_solve(spk_csr A, spk_precond M_left, spk_precond M_right,
                            const spk_solver_config* cfg, const void* d_b,
                            void* d_x, spk_summary* summary, double* history) {
   return guarded([&] {
+    TraceRange trace_("spk_gmres_solve");
     check_solver_args(A, M_left, M_right, cfg, d_b, d_x);
     SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
     *summary = spk_summary{};
@@ -420,6 +422,7 @@ spk_status spk_fgmres_solve(spk_csr A, spk_precond M,
                             const spk_solver_config* cfg, const void* d_b,
                             void* d_x, spk_summary* summary, double* history) {
   return guarded([&] {
+    TraceRange trace_("spk_fgmres_solve");
     check_solver_args(A, M, nullptr, cfg, d_b, d_x);
     SPK_REQUIRE(summary, SPK_INVALID_ARGUMENT, "null summary");
     *summary = spk_summary{};

@@ -10,9 +10,25 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "sparkit_hip.h"
 
 namespace spk {
+
+// ------------------------------------------------------------- tracing
+
+// A named roctx range for the span of a scope: the solvers mark their
+// entry points and phases (setup, restart cycles, refinement steps,
+// reliable updates, the learned solve's check / solve / fallback) so that
+// `rocprofv3 --marker-trace` places the kernels of a trace under them.
+// Without a tool attached a range costs a call that returns at once.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // ------------------------------------------------------------- errors
 

@@ -350,15 +350,19 @@ void cg_reliable(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   for (;;) {
     // FP32 steps until an update is due or the budget is spent
     int64_t chunk = 4;
-    while (ran < cfg->max_inner) {
-      const int64_t e = std::min(cfg->max_inner - ran, chunk);
-      for (int64_t j = 0; j < e; ++j, ++issued) step();
-      h = st.download();
-      ran = h.iters;
-      if (h.hold) break;
-      chunk = std::min<int64_t>(chunk * 2, 64);
+    {
+      TraceRange steps_("reliable CG: FP32 steps");
+      while (ran < cfg->max_inner) {
+        const int64_t e = std::min(cfg->max_inner - ran, chunk);
+        for (int64_t j = 0; j < e; ++j, ++issued) step();
+        h = st.download();
+        ran = h.iters;
+        if (h.hold) break;
+        chunk = std::min<int64_t>(chunk * 2, 64);
+      }
     }
     // reliable update: x64 += xi, FP64 true residual, convergence test
+    TraceRange update_("reliable CG: FP64 update");
     launch_map(n, [=] __device__(int64_t i) { x[i] += (double)xi[i]; });
     launch_spmv(*A, (const double*)x, EpiRcgTrue{b, r, S}, s);
     launch_finish<1>(reduce_partials(1), 0, [=] __device__(const double*) {
@@ -423,6 +427,7 @@ void ir_solve(spk_csr_s* A, spk_precond_s* M, const spk_ir_config* cfg,
   inner.precision = SPK_F32;
   inner.orthogonalization = cfg->orthogonalization;
   for (int64_t k = 0;; ++k) {
+    TraceRange step_("refinement step");
     double rel = h.rnorm / h.bnorm;
     if (history) history[k] = rel;
     out->relres = rel;
@@ -476,6 +481,7 @@ extern "C" spk_status spk_ir_solve(spk_csr A, spk_precond M,
                                    double* d_x, spk_ir_summary* out,
                                    double* history) {
   return guarded([&] {
+    TraceRange trace_("spk_ir_solve");
     SPK_REQUIRE(A && cfg && d_b && d_x && out, SPK_INVALID_ARGUMENT,
                 "null argument");
     SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,

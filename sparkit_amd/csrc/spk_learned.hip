@@ -103,6 +103,7 @@ extern "C" spk_status spk_learned_solve(spk_csr A, spk_precond M,
                                         const double* d_b, double* d_x,
                                         spk_learned_summary* out) {
   return guarded([&] {
+    TraceRange trace_("spk_learned_solve");
     SPK_REQUIRE(A && cfg && d_b && d_x && out, SPK_INVALID_ARGUMENT,
                 "null argument");
     SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
@@ -150,7 +151,10 @@ extern "C" spk_status spk_learned_solve(spk_csr A, spk_precond M,
     if (!M) {
       out->reason |= SPK_REASON_NO_PRECONDITIONER;
     } else {
-      check_inverse(*M->M, &out->bad_values, &out->bad_diagonal);
+      {
+        TraceRange check_("learned solve: check M");
+        check_inverse(*M->M, &out->bad_values, &out->bad_diagonal);
+      }
       if (out->bad_values) out->reason |= SPK_REASON_BAD_VALUES;
       if (out->bad_diagonal) out->reason |= SPK_REASON_BAD_DIAGONAL;
     }
@@ -174,6 +178,7 @@ extern "C" spk_status spk_learned_solve(spk_csr A, spk_precond M,
         return true;
       };
       spk_summary sum{};
+      TraceRange solve_("learned solve: FGMRES");
       fgmres_monitored(A, M, &c, d_b, d_x, &sum, monitor);
       check_preconds(M, nullptr);
       out->learned_iterations = sum.computed_iterations;
@@ -193,6 +198,7 @@ extern "C" spk_status spk_learned_solve(spk_csr A, spk_precond M,
       return;
     }
 
+    TraceRange fallback_("learned solve: fallback");
     // the fallback starts from the better of x0 and the learned x
     if (!(std::isfinite(out->learned_relres) && out->learned_relres < rel0))
       SPK_HIP_TRY(hipMemcpyAsync(d_x, x0, vb, hipMemcpyDeviceToDevice, s));

@@ -913,6 +913,7 @@ spk_status spk_precond_inner_cg_iterations(spk_precond M, int64_t* total) {
 
 spk_status spk_precond_create_ilu0(spk_csr A, int modified, spk_precond* out) {
   return guarded([&] {
+    TraceRange trace_("spk_precond_create_ilu0");
     SPK_REQUIRE(A && out, SPK_INVALID_ARGUMENT, "null argument");
     SPK_REQUIRE(A->nrows == A->ncols, SPK_INVALID_ARGUMENT,
                 "ilu0 requires a square matrix");
@@ -1043,6 +1044,7 @@ spk_status spk_precond_destroy(spk_precond M) {
 spk_status spk_precond_apply(spk_precond M, int dtype, const void* d_in,
                              void* d_out) {
   return guarded([&] {
+    TraceRange trace_("spk_precond_apply");
     SPK_REQUIRE(M && d_in && d_out, SPK_INVALID_ARGUMENT, "null argument");
     if (dtype == SPK_F32) M->apply((const float*)d_in, (float*)d_out);
     else M->apply((const double*)d_in, (double*)d_out);

@@ -358,6 +358,7 @@ extern "C" spk_status spk_csr_pseudo_peripheral_node(spk_csr A, int64_t* node) {
 // reverse_cuthill_mckee (reordering.cpp:152-203)
 extern "C" spk_status spk_csr_rcm(spk_csr A, int64_t* d_perm) {
   return guarded([&] {
+    TraceRange trace_("spk_csr_rcm");
     SPK_REQUIRE(A && d_perm, SPK_INVALID_ARGUMENT, "null argument");
     Rcm R;
     R.setup(*A);

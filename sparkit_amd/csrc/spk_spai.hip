@@ -772,6 +772,7 @@ spk_status spk_spai_train(spk_spai_model m, const spk_csr* mats, int64_t count,
                           const spk_spai_train_config* cfg,
                           double* loss_history) {
   return guarded([&] {
+    TraceRange trace_("spk_spai_train");
     SPK_REQUIRE(m && mats && cfg && count > 0, SPK_INVALID_ARGUMENT,
                 "null argument");
     SPK_REQUIRE(cfg->epochs >= 0 && cfg->batch_matrices > 0 &&
@@ -830,6 +831,7 @@ spk_status spk_spai_train(spk_spai_model m, const spk_csr* mats, int64_t count,
 
 spk_status spk_spai_predict(spk_spai_model m, spk_csr A, spk_csr* M_out) {
   return guarded([&] {
+    TraceRange trace_("spk_spai_predict");
     SPK_REQUIRE(m && A && M_out, SPK_INVALID_ARGUMENT, "null argument");
     m->prepare(*A, true);
     spk_csr_s* M = clone_pattern(*A);
