@@ -301,6 +301,14 @@ class Learned_spai {
       throw std::invalid_argument("Learned_spai: wrong parameter count");
     check(spk_spai_model_set_params(h_.get(), p.data()), "spk_spai_model_set_params");
   }
+  // the training state (weights, Adam moments and step) to a checkpoint
+  // file and back; file errors throw std::runtime_error
+  void save(const std::string& path) const {
+    check(spk_spai_model_save(h_.get(), path.c_str()), "spk_spai_model_save");
+  }
+  void load(const std::string& path) {
+    check(spk_spai_model_load(h_.get(), path.c_str()), "spk_spai_model_load");
+  }
   Device_csr approximate_inverse(const Device_csr& A) const {
     spk_csr M = nullptr;
     check(spk_spai_predict(h_.get(), A.handle(), &M), "spk_spai_predict");

@@ -573,6 +573,17 @@ spk_status spk_spai_model_destroy(spk_spai_model m);
 spk_status spk_spai_model_get_params(spk_spai_model m, float* params);
 /* replaces the weights and resets the optimiser state */
 spk_status spk_spai_model_set_params(spk_spai_model m, const float* params);
+/* No reference counterpart (the reference has no checkpoint; its only
+ * resume is the warm-start x): the training state -- FP32 weights, Adam's
+ * two moment vectors and step count -- to a checkpoint file and back, so a
+ * loaded model predicts the same M and continues training exactly as the
+ * saved one would.  Binary, little-endian: "SPKSPAI" NUL, uint32 version
+ * (1), uint32 SPK_SPAI_NPARAM, int64 step, 3 x SPK_SPAI_NPARAM floats
+ * (weights, first and second moments, canonical order), 64-bit FNV-1a of
+ * the preceding bytes.  File errors, a wrong magic, version, size or hash:
+ * SPK_RUNTIME_ERROR, and the model is left as it was. */
+spk_status spk_spai_model_save(spk_spai_model m, const char* path);
+spk_status spk_spai_model_load(spk_spai_model m, const char* path);
 /* the 24 features of every row into a device array (n x 24, float) */
 spk_status spk_spai_features(spk_csr A, float* d_features);
 /* loss (1/n)||I - M A||_F^2 and its gradient (host array of

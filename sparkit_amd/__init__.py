@@ -523,6 +523,17 @@ class SpaiModel:
         assert p.size == SPAI_NPARAM
         call("spk_spai_model_set_params", self.h, p.ctypes.data_as(C.c_void_p))
 
+    def save(self, path: str):
+        """The training state (weights, Adam moments, step) to a checkpoint
+        file (spk_spai_model_save)."""
+        call("spk_spai_model_save", self.h, os.fsencode(path))
+
+    def load(self, path: str) -> "SpaiModel":
+        """Replaces this model's training state with a checkpoint's
+        (spk_spai_model_load); returns the model."""
+        call("spk_spai_model_load", self.h, os.fsencode(path))
+        return self
+
     def loss_grad(self, A: CsrMatrix):
         """((1/n)||I - M A||_F^2, gradient) for one matrix, no update."""
         loss = C.c_double()

@@ -240,6 +240,8 @@ _SIGS = {
     "spk_spai_model_destroy": ([vp], None),
     "spk_spai_model_get_params": ([vp, vp], None),
     "spk_spai_model_set_params": ([vp, vp], None),
+    "spk_spai_model_save": ([vp, C.c_char_p], None),
+    "spk_spai_model_load": ([vp, C.c_char_p], None),
     "spk_spai_features": ([vp, vp], None),
     "spk_spai_loss_grad": ([vp, vp, P_dbl, vp], None),
     "spk_spai_train": ([vp, vp, i64, C.POINTER(SpaiTrainConfig), vp], None),

@@ -34,6 +34,8 @@
 // master weights, then refreshes the bf16 copy.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <utility>
 #include <vector>
 
@@ -700,6 +702,88 @@ spk_status spk_spai_model_set_params(spk_spai_model m, const float* params) {
   return guarded([&] {
     SPK_REQUIRE(m && params, SPK_INVALID_ARGUMENT, "null argument");
     m->upload(params);
+  });
+}
+
+// Checkpoint file (little-endian): "SPKSPAI" + NUL, uint32 version (1),
+// uint32 parameter count, int64 Adam step, then the FP32 master weights,
+// Adam's first and second moments (SPK_SPAI_NPARAM floats each, canonical
+// order), then a 64-bit FNV-1a hash of everything before it.
+namespace {
+constexpr char kCkptMagic[8] = {'S', 'P', 'K', 'S', 'P', 'A', 'I', '\0'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr size_t kCkptHead = 8 + 4 + 4 + 8;
+constexpr size_t kCkptBytes = kCkptHead + 3 * (size_t)kNP * 4 + 8;
+
+uint64_t fnv1a(const unsigned char* p, size_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
+  return h;
+}
+}  // namespace
+
+spk_status spk_spai_model_save(spk_spai_model m, const char* path) {
+  return guarded([&] {
+    SPK_REQUIRE(m && path, SPK_INVALID_ARGUMENT, "null argument");
+    std::vector<unsigned char> buf(kCkptBytes);
+    unsigned char* q = buf.data();
+    std::memcpy(q, kCkptMagic, 8);
+    std::memcpy(q + 8, &kCkptVersion, 4);
+    const uint32_t np = (uint32_t)kNP;
+    std::memcpy(q + 12, &np, 4);
+    std::memcpy(q + 16, &m->step, 8);
+    hipStream_t s = stream();
+    const DevBuf<float>* parts[3] = {&m->master, &m->m1, &m->m2};
+    for (int k = 0; k < 3; ++k)
+      SPK_HIP_TRY(hipMemcpyAsync(q + kCkptHead + (size_t)k * kNP * 4, parts[k]->p,
+                                 (size_t)kNP * 4, hipMemcpyDeviceToHost, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t h = fnv1a(q, kCkptBytes - 8);
+    std::memcpy(q + kCkptBytes - 8, &h, 8);
+    FILE* f = std::fopen(path, "wb");
+    SPK_REQUIRE(f, SPK_RUNTIME_ERROR, std::string("cannot open ") + path);
+    const size_t w = std::fwrite(q, 1, kCkptBytes, f);
+    const int c = std::fclose(f);
+    SPK_REQUIRE(w == kCkptBytes && c == 0, SPK_RUNTIME_ERROR,
+                std::string("cannot write ") + path);
+  });
+}
+
+spk_status spk_spai_model_load(spk_spai_model m, const char* path) {
+  return guarded([&] {
+    SPK_REQUIRE(m && path, SPK_INVALID_ARGUMENT, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    SPK_REQUIRE(f, SPK_RUNTIME_ERROR, std::string("cannot open ") + path);
+    std::vector<unsigned char> buf(kCkptBytes + 1);
+    const size_t r = std::fread(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    const unsigned char* q = buf.data();
+    SPK_REQUIRE(r >= 16 && std::memcmp(q, kCkptMagic, 8) == 0, SPK_RUNTIME_ERROR,
+                std::string(path) + " is not a learned-SPAI checkpoint");
+    uint32_t ver = 0, np = 0;
+    std::memcpy(&ver, q + 8, 4);
+    std::memcpy(&np, q + 12, 4);
+    SPK_REQUIRE(ver == kCkptVersion && np == (uint32_t)kNP, SPK_RUNTIME_ERROR,
+                std::string(path) + ": unsupported checkpoint version or size");
+    SPK_REQUIRE(r == kCkptBytes, SPK_RUNTIME_ERROR,
+                std::string(path) + ": checkpoint has the wrong length");
+    uint64_t h = 0;
+    std::memcpy(&h, q + kCkptBytes - 8, 8);
+    SPK_REQUIRE(h == fnv1a(q, kCkptBytes - 8), SPK_RUNTIME_ERROR,
+                std::string(path) + ": checkpoint checksum mismatch");
+    int64_t step = 0;
+    std::memcpy(&step, q + 16, 8);
+    SPK_REQUIRE(step >= 0, SPK_RUNTIME_ERROR, std::string(path) + ": bad step");
+    // weights (and their packed copy) first, then the optimiser state that
+    // upload() resets
+    m->upload(reinterpret_cast<const float*>(q + kCkptHead));
+    hipStream_t s = stream();
+    SPK_HIP_TRY(hipMemcpyAsync(m->m1.p, q + kCkptHead + (size_t)kNP * 4,
+                               (size_t)kNP * 4, hipMemcpyHostToDevice, s));
+    SPK_HIP_TRY(hipMemcpyAsync(m->m2.p, q + kCkptHead + 2 * (size_t)kNP * 4,
+                               (size_t)kNP * 4, hipMemcpyHostToDevice, s));
+    SPK_HIP_TRY(hipStreamSynchronize(s));
+    m->step = step;
   });
 }
 

@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "sparkit/hip/sparkit_hip.hpp"
 
 namespace {
@@ -361,6 +363,17 @@ void test_learned_spai_and_batched() {
   CHECK(throws<std::invalid_argument>([&] {
     net.set_parameters(std::vector<float>(3, 0.f));
   }));
+  // checkpoint: a loaded model continues training exactly like the saved one
+  const std::string ck = "/tmp/spk_test_api_" + std::to_string(::getpid()) + ".spai";
+  net.save(ck);
+  sh::Learned_spai twin(77);
+  twin.load(ck);
+  CHECK(twin.parameters() == net.parameters());
+  net.train(mats, 1, 2e-3, 1, 9);
+  twin.train(mats, 1, 2e-3, 1, 9);
+  CHECK(twin.parameters() == net.parameters());
+  std::remove(ck.c_str());
+  CHECK(throws<std::runtime_error>([&] { twin.load(ck); }));
   // batched PCG: two systems
   auto B = sh::Batched_pcg::diffusion(5, 9, 4, 12);
   CHECK(B.count() == 5);
