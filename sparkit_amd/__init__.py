@@ -72,6 +72,15 @@ def device_info():
     return cus.value, buf.value.decode()
 
 
+def _gfx_target():
+    """The GPU target name (e.g. gfx950) from spk_device_info."""
+    name = device_info()[1]
+    for tok in name.replace("(", " ").replace(")", " ").replace(":", " ").split():
+        if tok.startswith("gfx"):
+            return tok
+    return name
+
+
 class Timer:
     """HIP events on the library stream (the stream every kernel uses)."""
 
@@ -635,6 +644,38 @@ class BatchedSystems:
     def tuning(self, v):
         t = _lib.BatchTuning(int(v[0]), int(v[1]), int(v[2]), 0)
         call("spk_batch_set_tuning", self.h, C.byref(t))
+
+    def save_tuning(self, path: str, tried=None):
+        """The batch's launch shape (as autotune chose it) to a small JSON
+        file keyed by the GPU target, with the timings of the shapes tried
+        (autotune's second result) when given."""
+        import json
+        block, wg, lds = self.tuning
+        doc = {"format": "sparkit_amd batched PCG tuning", "version": 1,
+               "gfx_target": _gfx_target(),
+               "tuning": {"block": block, "wg_per_cu": wg, "lds_rows": lds}}
+        if tried is not None:
+            doc["tried"] = [{"block": t[0], "wg_per_cu": t[1], "lds_rows": t[2],
+                             "ms": float(ms)} for t, ms in tried]
+        with open(path, "w") as f:
+            json.dump(doc, f, indent=1)
+
+    def load_tuning(self, path: str):
+        """Applies a saved launch shape; InvalidArgument when it was tuned on
+        another GPU target (or the file is not such a record)."""
+        import json
+        with open(path) as f:
+            doc = json.load(f)
+        if doc.get("format") != "sparkit_amd batched PCG tuning" or \
+                doc.get("version") != 1:
+            raise InvalidArgument(1, "load_tuning", f"{path}: not a tuning record")
+        if doc.get("gfx_target") != _gfx_target():
+            raise InvalidArgument(1, "load_tuning",
+                                  f"{path}: tuned on {doc.get('gfx_target')}, "
+                                  f"this GPU is {_gfx_target()}")
+        t = doc["tuning"]
+        self.tuning = (t["block"], t["wg_per_cu"], t["lds_rows"])
+        return self.tuning
 
     @staticmethod
     def _cfg(cfg):

@@ -102,6 +102,31 @@ def test_autotune(spk, batch):
     assert s["converged"] == COUNT
 
 
+def test_tuning_record_round_trip(spk, batch, tmp_path):
+    """The autotuner's choice as a small JSON record keyed by the GPU
+    target, applied to another batch; a record from another target or a
+    foreign file is rejected."""
+    import json
+    b = _rhs(spk, batch)
+    x = spk.DeviceVector.zeros(batch.total_rows)
+    cfg = spk.BatchConfig(1e-8, 1e-4, 30, 5000, spk.F32)
+    chosen, tried = batch.autotune(cfg, b, x, sample=8)
+    path = tmp_path / "tuning.json"
+    batch.save_tuning(str(path), tried)
+    doc = json.loads(path.read_text())
+    assert doc["gfx_target"].startswith("gfx") and len(doc["tried"]) == len(tried)
+    other = spk.BatchedSystems.diffusion(6, SEED + 1, SMIN, SMAX)
+    other.tuning = (256, 1, 0)
+    assert other.load_tuning(str(path)) == chosen == other.tuning
+    doc["gfx_target"] = "gfx000"
+    path.write_text(json.dumps(doc))
+    with pytest.raises(spk.InvalidArgument, match="tuned on gfx000"):
+        other.load_tuning(str(path))
+    path.write_text("{}")
+    with pytest.raises(spk.InvalidArgument, match="not a tuning record"):
+        other.load_tuning(str(path))
+
+
 def test_from_blocks_and_errors(O, spk):
     A1 = O.batched_diffusion(3, 0, 4, 4)
     A2 = O.tridiagonal_matrix(5, -1.0, 3.0, -1.0)
