@@ -436,6 +436,13 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                                                learning_rate=lr,
                                                shuffle_seed=7))
     spk.synchronize()
+    # the same network trained on the loss of the operator PCG applies,
+    # (1/n) ||I - S A||_F^2 with S = (M + M^T) / 2
+    model_so = spk.SpaiModel(1)
+    model_so.objective = spk.SPAI_LOSS_SYMMETRIC
+    hist_so, t_train_so = _timed(lambda: model_so.train(
+        mats, epochs=epochs, learning_rate=lr, shuffle_seed=7))
+    spk.synchronize()
     del mats
     A = spk.aniso_q1_2d(n_side, seed, 1.5)
     n = A.nrows
@@ -449,7 +456,10 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
                "steps": len(hist), "learning_rate": lr,
                "generate_s": round(t_gen, 2), "train_s": round(t_train / 1e3, 3),
                "loss_first_epoch": float(np.mean(hist[:train_count])),
-               "loss_last_epoch": float(np.mean(hist[-train_count:]))},
+               "loss_last_epoch": float(np.mean(hist[-train_count:])),
+               "symmetric_objective_train_s": round(t_train_so / 1e3, 3),
+               "symmetric_objective_loss_last_epoch":
+                   float(np.mean(hist_so[-train_count:]))},
            "predict_ms": round(t_pred, 2),
            "loss_test_learned": model.loss_grad(A)[0],
            "loss_test_jacobi": untrained.loss_grad(A)[0]}
@@ -504,6 +514,13 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
     Ms16 = spk.spai(Msym16)
     run("learned_sym_relcg_fp32_update_every_10x_m_fp16",
         lambda x: spk.ir_solve(b, x, rcfg1, A, Ms16))
+    # the network trained on the symmetric part's own loss
+    Mso = spk.spai(model_so.predict(A).symmetric_part())
+    run("learned_sym_pcg_fp64_symmetric_objective",
+        lambda x: spk.conjugate_gradient(b, x, cg, A, Mso))
+    Mso16_st = model_so.predict(A).symmetric_part()
+    Mso16_st.set_half_values(True)
+    Mso16 = spk.spai(Mso16_st)
     for k in ("learned_sym_pcg_fp64", "jacobi_relcg_fp32", "learned_sym_relcg_fp32",
               "learned_sym_relcg_fp32_update_every_10x",
               "learned_sym_relcg_fp32_update_every_10x_m_fp16"):
@@ -563,7 +580,8 @@ def learned_spai_section(spk, n_side=1024, train_side=256, train_count=512,
         icfg = spk.IrConfig(tolerance=tol, max_outer=200, inner_tolerance=1e-1,
                             restart=0, max_inner=50000,
                             inner_method=spk.IR_CG_RELIABLE)
-        for tag, Mi in (("learned", Ms), ("learned_m_fp16", Ms16)):
+        for tag, Mi in (("learned", Ms), ("learned_m_fp16", Ms16),
+                        ("learned_symmetric_objective_m_fp16", Mso16)):
             P = spk.inner_ir(A, Mi, icfg)
             name = f"fgmres30_inner_relcg_to_{tol:g}_{tag}_fp32"
             run(name, lambda x: spk.fgmres(b, x, fg, A, P))

@@ -301,6 +301,12 @@ class Learned_spai {
       throw std::invalid_argument("Learned_spai: wrong parameter count");
     check(spk_spai_model_set_params(h_.get(), p.data()), "spk_spai_model_set_params");
   }
+  // training loss: SPK_SPAI_LOSS_INVERSE (||I - M A||, the default) or
+  // SPK_SPAI_LOSS_SYMMETRIC (||I - S A||, S = (M + M^T) / 2)
+  void set_objective(int objective) {
+    check(spk_spai_model_set_objective(h_.get(), objective),
+          "spk_spai_model_set_objective");
+  }
   // the training state (weights, Adam moments and step) to a checkpoint
   // file and back; file errors throw std::runtime_error
   void save(const std::string& path) const {

@@ -584,6 +584,16 @@ spk_status spk_spai_model_set_params(spk_spai_model m, const float* params);
  * SPK_RUNTIME_ERROR, and the model is left as it was. */
 spk_status spk_spai_model_save(spk_spai_model m, const char* path);
 spk_status spk_spai_model_load(spk_spai_model m, const char* path);
+/* The training objective of spk_spai_train and spk_spai_loss_grad (no
+ * reference counterpart): SPK_SPAI_LOSS_INVERSE (the default)
+ * (1/n) ||I - M A||_F^2, or SPK_SPAI_LOSS_SYMMETRIC
+ * (1/n) ||I - S A||_F^2 with S = (M + M^T) / 2 -- the operator CG applies
+ * (spk_csr_symmetric_part), for structurally symmetric patterns
+ * (SPK_INVALID_ARGUMENT otherwise).  Predictions are M either way.  Not
+ * part of a checkpoint. */
+enum { SPK_SPAI_LOSS_INVERSE = 0, SPK_SPAI_LOSS_SYMMETRIC = 1 };
+spk_status spk_spai_model_set_objective(spk_spai_model m, int objective);
+spk_status spk_spai_model_get_objective(spk_spai_model m, int* objective);
 /* the 24 features of every row into a device array (n x 24, float) */
 spk_status spk_spai_features(spk_csr A, float* d_features);
 /* loss (1/n)||I - M A||_F^2 and its gradient (host array of

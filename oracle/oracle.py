@@ -191,6 +191,10 @@ def _declare(L):
     L.orc_spai_loss_grad_bf16.restype = C.c_double
     L.orc_spai_loss_grad_exact.argtypes = [_f64p, C.POINTER(CsrD), C.c_void_p]
     L.orc_spai_loss_grad_exact.restype = C.c_double
+    L.orc_spai_sym_loss_grad_bf16.argtypes = [_f32p, C.POINTER(CsrD), C.c_void_p]
+    L.orc_spai_sym_loss_grad_bf16.restype = C.c_double
+    L.orc_spai_sym_loss_grad_exact.argtypes = [_f64p, C.POINTER(CsrD), C.c_void_p]
+    L.orc_spai_sym_loss_grad_exact.restype = C.c_double
     L.orc_spai_predict_bf16.argtypes = [_f32p, C.POINTER(CsrD), _f64p]
     L.orc_spai_predict_exact.argtypes = [_f64p, C.POINTER(CsrD), _f64p]
 
@@ -925,16 +929,19 @@ def spai_features(A):
     return X
 
 
-def spai_loss_grad(params, A, exact=False):
-    """(loss, gradient) of (1/n)||I - M A||_F^2; bf16 device arithmetic
+def spai_loss_grad(params, A, exact=False, symmetric=False):
+    """(loss, gradient) of (1/n)||I - M A||_F^2, or with symmetric of
+    (1/n)||I - S A||_F^2 for S = (M + M^T) / 2 (no reference counterpart:
+    the objective of the operator CG applies); bf16 device arithmetic
     unless exact (float64 throughout)."""
     g = np.zeros(SPAI_NPARAM, np.float64)
+    name = "orc_spai_sym_loss_grad" if symmetric else "orc_spai_loss_grad"
     if exact:
         P = np.ascontiguousarray(params, np.float64)
-        loss = lib().orc_spai_loss_grad_exact(P, C.byref(A._struct()), _ptr(g))
+        loss = getattr(lib(), name + "_exact")(P, C.byref(A._struct()), _ptr(g))
     else:
         P = np.ascontiguousarray(params, np.float32)
-        loss = lib().orc_spai_loss_grad_bf16(P, C.byref(A._struct()), _ptr(g))
+        loss = getattr(lib(), name + "_bf16")(P, C.byref(A._struct()), _ptr(g))
     return loss, g
 
 

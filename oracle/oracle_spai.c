@@ -143,6 +143,15 @@ static double spai_row_residual(const orc_csr_d* A, int64_t i,
   return loss;
 }
 
+/* position of entry (j, i) for entry p = (i, j), by a linear scan of row j;
+ * -1 when the pattern has no such entry */
+static int64_t spai_mirror(const orc_csr_d* A, int64_t i, int64_t p) {
+  const int64_t j = A->col_ind[p];
+  for (int64_t q = A->row_ptr[j]; q < A->row_ptr[j + 1]; ++q)
+    if (A->col_ind[q] == i) return q;
+  return -1;
+}
+
 int orc_spai_features(const orc_csr_d* A, float* X) {
   double *diag, *rsum;
   int st = spai_prepare(A, &diag, &rsum);

@@ -417,6 +417,12 @@ double orc_spai_loss_grad_exact(const double* params, const orc_csr_d* A,
                                 double* grad);
 int orc_spai_predict_bf16(const float* params, const orc_csr_d* A,
                           double* mvals);
+/* the symmetric objective (1/n) ||I - S A||_F^2, S = (M + M^T) / 2, and its
+ * gradient; NaN also for a pattern that is not structurally symmetric */
+double orc_spai_sym_loss_grad_bf16(const float* params, const orc_csr_d* A,
+                                   double* grad);
+double orc_spai_sym_loss_grad_exact(const double* params, const orc_csr_d* A,
+                                    double* grad);
 int orc_spai_predict_exact(const double* params, const orc_csr_d* A,
                            double* mvals);
 

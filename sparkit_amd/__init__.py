@@ -28,7 +28,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS2, ORTHO_MGS, InvalidArgument,
+from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS2, ORTHO_MGS, SPAI_LOSS_INVERSE,
+                   SPAI_LOSS_SYMMETRIC, InvalidArgument,
                    LogicError, ReadError, SpkError, call, load)
 
 __all__ = [
@@ -41,7 +42,7 @@ __all__ = [
     "true_relres", "dot", "axpby", "inner_cg", "inner_ir", "InnerCg", "learned_solve", "LearnedConfig",
     "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
     "FALLBACK_ILU_GMRES", "FALLBACK_NONE",
-    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
+    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SPAI_LOSS_INVERSE", "SPAI_LOSS_SYMMETRIC", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB", "IR_CG_RELIABLE",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
@@ -531,6 +532,17 @@ class SpaiModel:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == SPAI_NPARAM
         call("spk_spai_model_set_params", self.h, p.ctypes.data_as(C.c_void_p))
+
+    @property
+    def objective(self) -> int:
+        """SPAI_LOSS_INVERSE or SPAI_LOSS_SYMMETRIC (the training loss)."""
+        v = C.c_int32()
+        call("spk_spai_model_get_objective", self.h, C.byref(v))
+        return v.value
+
+    @objective.setter
+    def objective(self, v: int):
+        call("spk_spai_model_set_objective", self.h, int(v))
 
     def save(self, path: str):
         """The training state (weights, Adam moments, step) to a checkpoint

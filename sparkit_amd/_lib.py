@@ -24,6 +24,8 @@ F32 = 1
 
 ORTHO_MGS = 0
 ORTHO_CGS2 = 1
+SPAI_LOSS_INVERSE = 0    # (1/n) ||I - M A||_F^2
+SPAI_LOSS_SYMMETRIC = 1  # (1/n) ||I - S A||_F^2, S = (M + M^T) / 2
 
 
 class SpkError(RuntimeError):
@@ -240,6 +242,8 @@ _SIGS = {
     "spk_spai_model_destroy": ([vp], None),
     "spk_spai_model_get_params": ([vp, vp], None),
     "spk_spai_model_set_params": ([vp, vp], None),
+    "spk_spai_model_set_objective": ([vp, i32], None),
+    "spk_spai_model_get_objective": ([vp, C.POINTER(i32)], None),
     "spk_spai_model_save": ([vp, C.c_char_p], None),
     "spk_spai_model_load": ([vp, C.c_char_p], None),
     "spk_spai_features": ([vp, vp], None),

@@ -139,12 +139,11 @@ struct BatchArgs {
   int ell_w;
 };
 
-// sliced-ELL widths compiled (rows are padded to the next one; wider rows
-// keep the CSR loop); rows of the inner loop per thread and pass (their
-// loads are all issued before the first use)
+// sliced-ELL widths compiled, 3, 5, 7 and 9 (rows are padded to the next
+// one; wider rows keep the CSR loop)
 constexpr int kEllMax = 9;
 inline int ell_width_for(int w) {
-  return w <= 3 ? 3 : w <= 5 ? 5 : w <= 7 ? 7 : w <= 9 ? 9 : 0;
+  return w <= 3 ? 3 : w <= 5 ? 5 : w <= 7 ? 7 : w <= kEllMax ? kEllMax : 0;
 }
 constexpr int kBatchRows = 2;
 
@@ -206,8 +205,7 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
     const int64_t r0 = a.sys_row[sid];
     const int n = (int)(a.sys_row[sid + 1] - r0);
     const bool in_lds = n <= a.lds_rows;
-    // p through LDS (small systems) or global memory
-    auto pget = [&](int64_t g) -> T { return in_lds ? lds_p[g - r0] : P[g]; };
+    // p stored through LDS (small systems) or global memory
     auto pset = [&](int64_t g, T v) {
       if (in_lds) lds_p[g - r0] = v; else P[g] = v;
     };

@@ -143,7 +143,72 @@ struct TileArgs {
   double* __restrict__ loss_part;  // train: [gridDim.x]
   double scale;                  // train: 1 / rows of the batch
   int accumulate;                // train: add to the existing partials
+  // train, symmetric objective: dL/dm per entry of A's pattern and each
+  // row's loss, computed beforehand (null: the row merge below)
+  const double* __restrict__ gm;
+  const double* __restrict__ rowloss;
 };
+
+// Row i's residual e_i - sum_s m_s A_{j_s,:} by a merge of the slot rows
+// (their columns ascend): returns ||.||^2 and adds dL/dm_s to gs[s].
+__device__ __forceinline__ double row_residual(const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ ci,
+                                               const double* __restrict__ v,
+                                               int64_t i, const int* pos,
+                                               const double* m, double* gs) {
+#pragma clang fp contract(off)
+  int hp[kO], he[kO], hc[kO];
+  double hv[kO];
+#pragma unroll
+  for (int s = 0; s < kO; ++s) {
+    hp[s] = he[s] = 0;
+    if (pos[s] >= 0) {
+      const int j = ci[pos[s]];
+      hp[s] = rp[j];
+      he[s] = rp[j + 1];
+    }
+    hc[s] = hp[s] < he[s] ? ci[hp[s]] : INT32_MAX;
+    hv[s] = hp[s] < he[s] ? v[hp[s]] : 0.0;
+  }
+  double lrow = 0.0;
+  for (;;) {
+    int c = INT32_MAX;
+#pragma unroll
+    for (int s = 0; s < kO; ++s) c = hc[s] < c ? hc[s] : c;
+    if (c == INT32_MAX) break;
+    double tsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < kO; ++s)
+      if (hc[s] == c) tsum += m[s] * hv[s];
+    const double res = ((int64_t)c == i ? 1.0 : 0.0) - tsum;
+    lrow += res * res;
+#pragma unroll
+    for (int s = 0; s < kO; ++s)
+      if (hc[s] == c) {
+        gs[s] += -2.0 * res * hv[s];
+        ++hp[s];
+        hc[s] = hp[s] < he[s] ? ci[hp[s]] : INT32_MAX;
+        hv[s] = hp[s] < he[s] ? v[hp[s]] : 0.0;
+      }
+  }
+  return lrow;
+}
+
+// Row i's slot positions: pos[4] its diagonal, pos[3..0] the entries left
+// of it from the nearest outwards, pos[5..8] those right of it (-1 unused).
+__device__ __forceinline__ void row_slots(const int32_t* __restrict__ rp,
+                                          const int32_t* __restrict__ ci, int64_t i,
+                                          int* pos) {
+  const int p0 = rp[i], p1 = rp[i + 1];
+  int d = p0;
+  while (ci[d] != (int32_t)i) ++d;  // present: checked by the prepare pass
+  pos[4] = d;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    pos[3 - k] = (d - 1 - k >= p0) ? d - 1 - k : -1;
+    pos[5 + k] = (d + 1 + k < p1) ? d + 1 + k : -1;
+  }
+}
 
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void spai_tile_kernel(TileArgs A) {
@@ -200,15 +265,7 @@ __global__ __launch_bounds__(256) void spai_tile_kernel(TileArgs A) {
 #pragma unroll
     for (int f = 0; f < kF; ++f) x[f] = 0.f;
     if (valid) {
-      const int p0 = A.rp[i], p1 = A.rp[i + 1];
-      int d = p0;
-      while (A.ci[d] != (int32_t)i) ++d;  // present: checked by the prepare pass
-      pos[4] = d;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        pos[3 - k] = (d - 1 - k >= p0) ? d - 1 - k : -1;
-        pos[5 + k] = (d + 1 + k < p1) ? d + 1 + k : -1;
-      }
+      row_slots(A.rp, A.ci, i, pos);
       di = A.diag[i];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -281,42 +338,20 @@ __global__ __launch_bounds__(256) void spai_tile_kernel(TileArgs A) {
       continue;
     }
 
-    // ---- (M A)_i by a merge of the slot rows; g_s = dL/dm_s
+    // ---- (M A)_i by a merge of the slot rows; g_s = dL/dm_s (or both
+    // from the symmetric objective's passes)
     double gs[kO];
-    int hp[kO], he[kO], hc[kO];
-    double hv[kO];
 #pragma unroll
-    for (int s = 0; s < kO; ++s) {
-      gs[s] = 0.0;
-      hp[s] = he[s] = 0;
-      if (pos[s] >= 0) {
-        const int j = A.ci[pos[s]];
-        hp[s] = A.rp[j];
-        he[s] = A.rp[j + 1];
-      }
-      hc[s] = hp[s] < he[s] ? A.ci[hp[s]] : INT32_MAX;
-      hv[s] = hp[s] < he[s] ? A.v[hp[s]] : 0.0;
-    }
+    for (int s = 0; s < kO; ++s) gs[s] = 0.0;
     double lrow = 0.0;
-    for (;;) {
-      int c = INT32_MAX;
+    if (A.gm) {
+      if (valid) {
 #pragma unroll
-      for (int s = 0; s < kO; ++s) c = hc[s] < c ? hc[s] : c;
-      if (c == INT32_MAX) break;
-      double tsum = 0.0;
-#pragma unroll
-      for (int s = 0; s < kO; ++s)
-        if (hc[s] == c) tsum += m[s] * hv[s];
-      const double res = ((int64_t)c == i ? 1.0 : 0.0) - tsum;
-      lrow += res * res;
-#pragma unroll
-      for (int s = 0; s < kO; ++s)
-        if (hc[s] == c) {
-          gs[s] += -2.0 * res * hv[s];
-          ++hp[s];
-          hc[s] = hp[s] < he[s] ? A.ci[hp[s]] : INT32_MAX;
-          hv[s] = hp[s] < he[s] ? A.v[hp[s]] : 0.0;
-        }
+        for (int s = 0; s < kO; ++s) gs[s] = pos[s] >= 0 ? A.gm[pos[s]] : 0.0;
+        lrow = A.rowloss[i];
+      }
+    } else {
+      lrow = row_residual(A.rp, A.ci, A.v, i, pos, m, gs);
     }
     if (valid) lsum += lrow;
     float dy[kO];
@@ -465,6 +500,66 @@ __global__ __launch_bounds__(256) void spai_tile_kernel(TileArgs A) {
     dst[k] = (A.accumulate ? dst[k] : 0.f) + red[k];
 }
 
+// ---- the symmetric objective (1/n) ||I - S A||_F^2, S = (M + M^T) / 2:
+// with mirror(p) the position of entry (j, i) for entry p = (i, j),
+// S_p = (m_p + m_mirror(p)) / 2, each row's residual and g_p = dL/dS_p come
+// from row_residual over S's values, and dL/dm_p = (g_p + g_mirror(p)) / 2
+// (an entry of M enters S twice, at p and at its mirror).
+
+// mirror positions, one thread per row; a missing mirror (a pattern that
+// is not structurally symmetric) sets *err
+__global__ void spai_mirror_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                   const int32_t* __restrict__ ci,
+                                   int32_t* __restrict__ mirror, int* err) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = rp[i]; p < rp[i + 1]; ++p) {
+    const int j = ci[p];
+    int lo = rp[j], hi = rp[j + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ci[mid] < (int32_t)i) lo = mid + 1;
+      else hi = mid;
+    }
+    const bool ok = lo < rp[j + 1] && ci[lo] == (int32_t)i;
+    if (!ok) atomicOr(err, 1);
+    mirror[p] = ok ? lo : p;
+  }
+}
+
+// out_p = (in_p + in_mirror(p)) / 2 over the entries
+__global__ void spai_mirror_mean_kernel(int64_t nnz, const double* __restrict__ in,
+                                        const int32_t* __restrict__ mirror,
+                                        double* __restrict__ out) {
+#pragma clang fp contract(off)
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz;
+       p += (int64_t)gridDim.x * blockDim.x)
+    out[p] = 0.5 * (in[p] + in[mirror[p]]);
+}
+
+// each row's residual under S's values: the row's loss and g_p = dL/dS_p
+__global__ void spai_row_loss_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                     const int32_t* __restrict__ ci,
+                                     const double* __restrict__ v,
+                                     const double* __restrict__ svals,
+                                     double* __restrict__ g,
+                                     double* __restrict__ rowloss) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int pos[kO];
+  row_slots(rp, ci, i, pos);
+  double m[kO], gs[kO];
+#pragma unroll
+  for (int s = 0; s < kO; ++s) {
+    m[s] = pos[s] >= 0 ? svals[pos[s]] : 0.0;
+    gs[s] = 0.0;
+  }
+  rowloss[i] = row_residual(rp, ci, v, i, pos, m, gs);
+#pragma unroll
+  for (int s = 0; s < kO; ++s)
+    if (pos[s] >= 0) g[pos[s]] = gs[s];
+}
+
 // canonical parameter k -> (gradient partial index, packed slot)
 __device__ __forceinline__ int partial_index(int k) {
   if (k < kB1) return gW1 + k;  // [f][o], same row length 64
@@ -541,6 +636,12 @@ struct spk_spai_model_s {
   DevBuf<int> err;
   int64_t step = 0;
   int nparts = 0;
+  // training objective: SPK_SPAI_LOSS_INVERSE (||I - M A||) or
+  // SPK_SPAI_LOSS_SYMMETRIC (||I - S A||, S = (M + M^T) / 2), with the
+  // latter's per-matrix scratch on A's pattern
+  int objective = SPK_SPAI_LOSS_INVERSE;
+  DevBuf<int32_t> mirror;
+  DevBuf<double> mv, sv, gsv, gmv, rowl;
 
   void reset_optimizer() {
     hipStream_t s = stream();
@@ -595,8 +696,74 @@ struct spk_spai_model_s {
     t.pk = pk.p;
     return t;
   }
+  // grid of a predicting launch over A's row tiles
+  int predict_grid(const spk_csr_s& A) const {
+    const int64_t ntiles = (A.nrows + kTile - 1) / kTile;
+    const int64_t want = (ntiles + kWaves - 1) / kWaves;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, nparts));
+  }
+  // M's values on A's pattern (every entry is a slot: prepare checked)
+  void predict_into(spk_csr_s& A, double* mvals) {
+    TileArgs t = args(A);
+    t.mvals = mvals;
+    hipLaunchKernelGGL(spai_tile_kernel<false>, dim3(predict_grid(A)),
+                       dim3(kWaves * 64), 0, stream(), t);
+    SPK_HIP_TRY(hipGetLastError());
+  }
+  // The symmetric objective's passes before the backward one: M, the
+  // mirrors, S, each row's loss and dL/dS, then dL/dm (into gmv / rowl).
+  // check: report a pattern that is not structurally symmetric.
+  void symmetric_scratch(const spk_csr_s& A) {
+    const size_t nz = (size_t)(A.nnz > 0 ? A.nnz : 1), n = (size_t)A.nrows;
+    if (mirror.n < nz) {
+      mirror.alloc(nz);
+      mv.alloc(nz);
+      sv.alloc(nz);
+      gsv.alloc(nz);
+      gmv.alloc(nz);
+    }
+    if (rowl.n < n) rowl.alloc(n);
+  }
+  // the mirror positions of A's entries; check: SPK_INVALID_ARGUMENT for a
+  // pattern that is not structurally symmetric
+  void mirrors(const spk_csr_s& A, bool check) {
+    symmetric_scratch(A);
+    hipStream_t s = stream();
+    if (check) SPK_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(spai_mirror_kernel, dim3((int)ceil_div(A.nrows, 256)),
+                       dim3(256), 0, s, A.nrows, (const int32_t*)A.rp.p,
+                       (const int32_t*)A.ci.p, mirror.p, err.p);
+    SPK_HIP_TRY(hipGetLastError());
+    if (check) {
+      int h = 0;
+      SPK_HIP_TRY(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      SPK_REQUIRE(!h, SPK_INVALID_ARGUMENT,
+                  "the symmetric objective needs a structurally symmetric pattern");
+    }
+  }
+  void symmetric_grad(spk_csr_s& A) {
+    hipStream_t s = stream();
+    predict_into(A, mv.p);
+    mirrors(A, false);
+    const int gr = (int)ceil_div(A.nrows, 256);
+    const int ge = (int)std::min<int64_t>(ceil_div(A.nnz > 0 ? A.nnz : 1, 256), 65536);
+    hipLaunchKernelGGL(spai_mirror_mean_kernel, dim3(ge), dim3(256), 0, s, A.nnz,
+                       (const double*)mv.p, (const int32_t*)mirror.p, sv.p);
+    hipLaunchKernelGGL(spai_row_loss_kernel, dim3(gr), dim3(256), 0, s, A.nrows,
+                       (const int32_t*)A.rp.p, (const int32_t*)A.ci.p,
+                       (const double*)A.v64.p, (const double*)sv.p, gsv.p, rowl.p);
+    hipLaunchKernelGGL(spai_mirror_mean_kernel, dim3(ge), dim3(256), 0, s, A.nnz,
+                       (const double*)gsv.p, (const int32_t*)mirror.p, gmv.p);
+    SPK_HIP_TRY(hipGetLastError());
+  }
   void train_tiles(spk_csr_s& A, double scale, bool accumulate) {
     TileArgs t = args(A);
+    if (objective == SPK_SPAI_LOSS_SYMMETRIC) {
+      symmetric_grad(A);
+      t.gm = gmv.p;
+      t.rowloss = rowl.p;
+    }
     t.partials = partials.p;
     t.loss_part = loss_part.p;
     t.scale = scale;
@@ -787,6 +954,23 @@ spk_status spk_spai_model_load(spk_spai_model m, const char* path) {
   });
 }
 
+spk_status spk_spai_model_set_objective(spk_spai_model m, int objective) {
+  return guarded([&] {
+    SPK_REQUIRE(m, SPK_INVALID_ARGUMENT, "null argument");
+    SPK_REQUIRE(objective == SPK_SPAI_LOSS_INVERSE ||
+                    objective == SPK_SPAI_LOSS_SYMMETRIC,
+                SPK_INVALID_ARGUMENT, "unknown objective");
+    m->objective = objective;
+  });
+}
+
+spk_status spk_spai_model_get_objective(spk_spai_model m, int* objective) {
+  return guarded([&] {
+    SPK_REQUIRE(m && objective, SPK_INVALID_ARGUMENT, "null argument");
+    *objective = m->objective;
+  });
+}
+
 spk_status spk_spai_features(spk_csr A, float* d_features) {
   return guarded([&] {
     SPK_REQUIRE(A && d_features, SPK_INVALID_ARGUMENT, "null argument");
@@ -838,6 +1022,7 @@ spk_status spk_spai_loss_grad(spk_spai_model m, spk_csr A, double* loss,
   return guarded([&] {
     SPK_REQUIRE(m && A && loss, SPK_INVALID_ARGUMENT, "null argument");
     m->prepare(*A, true);
+    if (m->objective == SPK_SPAI_LOSS_SYMMETRIC) m->mirrors(*A, true);
     m->train_tiles(*A, 1.0 / (double)A->nrows, false);
     hipStream_t s = stream();
     hipLaunchKernelGGL(spai_update_kernel, dim3((kNP + 255) / 256), dim3(256), 0,
@@ -865,6 +1050,7 @@ spk_status spk_spai_train(spk_spai_model m, const spk_csr* mats, int64_t count,
     for (int64_t k = 0; k < count; ++k) {
       SPK_REQUIRE(mats[k], SPK_INVALID_ARGUMENT, "null matrix");
       m->prepare(*mats[k], true);
+      if (m->objective == SPK_SPAI_LOSS_SYMMETRIC) m->mirrors(*mats[k], true);
     }
     const int64_t per_epoch = (count + cfg->batch_matrices - 1) / cfg->batch_matrices;
     const int64_t total = cfg->epochs * per_epoch;
@@ -920,14 +1106,7 @@ spk_status spk_spai_predict(spk_spai_model m, spk_csr A, spk_csr* M_out) {
     m->prepare(*A, true);
     spk_csr_s* M = clone_pattern(*A);
     try {
-      TileArgs t = m->args(*A);
-      t.mvals = M->v64.p;
-      const int64_t ntiles = (A->nrows + kTile - 1) / kTile;
-      const int64_t want = (ntiles + kWaves - 1) / kWaves;
-      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, m->nparts));
-      hipLaunchKernelGGL(spai_tile_kernel<false>, dim3(grid), dim3(kWaves * 64), 0,
-                         stream(), t);
-      SPK_HIP_TRY(hipGetLastError());
+      m->predict_into(*A, M->v64.p);
       SPK_HIP_TRY(hipStreamSynchronize(stream()));
     } catch (...) {
       delete M;

@@ -128,3 +128,66 @@ def test_training_helps_fgmres(O, spk):
         # the trained network's loss on this unseen matrix is lower too
         assert model.loss_grad(A)[0] < jac.loss_grad(A)[0]
     assert min(gains) > 1.1, gains
+
+
+# ---- the symmetric objective (1/n) ||I - S A||_F^2, S = (M + M^T) / 2
+# (spk_spai_model_set_objective; no reference counterpart)
+
+
+@pytest.mark.parametrize("N", [13, 64])
+def test_symmetric_loss_and_gradient_match_oracle(O, spk, N):
+    p = rand_params(6, scale3=0.2)
+    model = spk.SpaiModel(0)
+    model.params = p
+    model.objective = spk.SPAI_LOSS_SYMMETRIC
+    assert model.objective == spk.SPAI_LOSS_SYMMETRIC
+    A = spk.aniso_q1_2d(N, 9, 1.5)
+    R = O.aniso_q1_2d(N, 9, 1.5)
+    loss, g = model.loss_grad(A)
+    lr, gr = O.spai_loss_grad(p, R, symmetric=True)
+    assert abs(loss - lr) <= 1e-3 * lr
+    for name, sl in [("W1", slice(0, 1536)), ("b1", slice(1536, 1600)),
+                     ("W2", slice(1600, 5696)), ("b2", slice(5696, 5760)),
+                     ("W3", slice(5760, 6336)), ("b3", slice(6336, 6345))]:
+        d = np.linalg.norm(g[sl] - gr[sl])
+        assert d <= 2e-2 * np.linalg.norm(gr[sl]) + 1e-7, name
+    loss2, g2 = model.loss_grad(A)
+    assert loss2 == loss and np.array_equal(g2, g)
+    # the other objective on the same model gives the plain loss again
+    model.objective = spk.SPAI_LOSS_INVERSE
+    assert abs(model.loss_grad(A)[0] - O.spai_loss_grad(p, R)[0]) <= 1e-3 * lr
+
+
+def test_symmetric_objective_rejects_nonsymmetric_pattern(O, spk):
+    A = spk.CsrMatrix.from_oracle(
+        O.make_matrix(3, 3, [0, 0, 1, 2], [0, 1, 1, 2], [2.0, -1.0, 2.0, 2.0]))
+    model = spk.SpaiModel(0)
+    model.objective = spk.SPAI_LOSS_SYMMETRIC
+    with pytest.raises(spk.InvalidArgument, match="structurally symmetric"):
+        model.loss_grad(A)
+    with pytest.raises(spk.InvalidArgument, match="structurally symmetric"):
+        model.train([A], epochs=1)
+    with pytest.raises(spk.InvalidArgument):
+        model.objective = 7
+
+
+def test_symmetric_objective_trains_the_symmetric_part(spk):
+    """Trained on S's own loss, the network reaches a lower S-loss than the
+    same training on the plain loss, and its S preconditions CG."""
+    mats = [spk.aniso_q1_2d(48, s, 1.5) for s in range(16)]
+    plain, sym = spk.SpaiModel(1), spk.SpaiModel(1)
+    sym.objective = spk.SPAI_LOSS_SYMMETRIC
+    for m in (plain, sym):
+        h = m.train(mats, epochs=6, learning_rate=2e-3, shuffle_seed=5)
+        assert np.all(np.isfinite(h))
+    A = spk.aniso_q1_2d(64, 1000, 1.5)
+    plain.objective = spk.SPAI_LOSS_SYMMETRIC
+    assert sym.loss_grad(A)[0] < plain.loss_grad(A)[0]
+    b = spk.uniform_pm1(7, A.nrows)
+    its = []
+    for M in (spk.jacobi(A), spk.spai(sym.predict(A).symmetric_part())):
+        x = spk.DeviceVector.zeros(A.nrows)
+        s = spk.conjugate_gradient(b, x, spk.CGConfig(1e-8, 20000, 20000), A, M)
+        assert s.converged and spk.true_relres(A, b, x) < 1e-7
+        its.append(s.computed_iterations)
+    assert its[1] < 0.8 * its[0], its

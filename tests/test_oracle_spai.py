@@ -135,3 +135,59 @@ def test_rejects_rows_beyond_slots(O):
     A = O.make_matrix(n, n, rows, cols, [4.0] + [-0.1] * 6 + [1.0] * (n - 1))
     with pytest.raises(O.OracleError):
         O.spai_features(A)
+
+
+# ---- the symmetric objective (1/n) ||I - S A||_F^2, S = (M + M^T) / 2 (no
+# reference counterpart), pinned like the plain one
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_symmetric_loss_matches_dense(O, small, exact):
+    p = rand_params(5, scale3=0.2)
+    m = O.spai_predict(p, small, exact=exact)
+    M = small.__class__(small.nrows, small.ncols, small.row_ptr,
+                        small.col_ind, m).dense()
+    S = 0.5 * (M + M.T)
+    R = np.eye(small.nrows) - S @ small.dense()
+    loss, _ = O.spai_loss_grad(p, small, exact=exact, symmetric=True)
+    assert abs(loss - (R ** 2).sum() / small.nrows) <= 1e-11 * loss
+    # a genuinely nonsymmetric M: the two objectives differ
+    assert abs(loss - O.spai_loss_grad(p, small, exact=exact)[0]) > 1e-6 * loss
+
+
+def test_symmetric_gradient_matches_finite_differences(O, small):
+    p = rand_params(6, scale3=0.2)
+    _, g = O.spai_loss_grad(p, small, exact=True, symmetric=True)
+    rng = np.random.default_rng(1)
+    idx = list(rng.choice(6345, 40, replace=False)) + list(np.argsort(-abs(g))[:10])
+    for k in idx:
+        h = 1e-6 * max(1.0, abs(p[k]))
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (O.spai_loss_grad(pp, small, exact=True, symmetric=True)[0] -
+              O.spai_loss_grad(pm, small, exact=True, symmetric=True)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-5 * np.abs(g).max() + 1e-6 * abs(g[k]), k
+
+
+def test_symmetric_loss_equals_plain_for_symmetric_m(O, small):
+    """A zero output layer predicts M = D^-1, already symmetric: S = M bit
+    for bit, so the two losses agree exactly (their gradients do not: the
+    symmetric one averages each entry's with its mirror's)."""
+    p = rand_params(7)
+    u = O.spai_unpack(p)
+    u["W3"][:] = 0
+    u["b3"][:] = 0
+    for exact in (True, False):
+        a = O.spai_loss_grad(p, small, exact=exact)
+        b = O.spai_loss_grad(p, small, exact=exact, symmetric=True)
+        assert a[0] == b[0]
+        assert not np.array_equal(a[1], b[1])
+
+
+def test_symmetric_loss_needs_symmetric_pattern(O):
+    # 3 x 3, diagonal plus one entry above it: (0, 1) has no mirror
+    A = O.make_matrix(3, 3, [0, 0, 1, 2], [0, 1, 1, 2], [2.0, -1.0, 2.0, 2.0])
+    p = rand_params(8)
+    assert np.isnan(O.spai_loss_grad(p, A, exact=True, symmetric=True)[0])
+    assert np.isfinite(O.spai_loss_grad(p, A, exact=True)[0])
