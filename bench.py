@@ -905,12 +905,14 @@ def batched_section(spk, count=4096, seed=42, smin=32, smax=256,
     tot = float(np.sum(it * bytes_it))
     gbs = tot / (ms32 * 1e-3) / 1e9
     # what the kernel moves through HBM per inner iteration and row: the
-    # sliced-ELL matrix (5 slots x (4 + 4) bytes), q written, the d/r pass
+    # sliced-ELL matrix (5 slots x (4-byte value + 2-byte column offset):
+    # every column of these grids lies within 256 of its row), q written,
+    # the d/r pass
     # (d, r, q, 1/diag read, d, r written) and the p pass (r, 1/diag read);
     # p itself (read by the gathers and both passes, written by the p pass)
     # stays in LDS for systems up to lds_rows rows, else adds 16 bytes
     lds_rows = chosen[2]
-    per_row = 5 * 8 + 4 + 24 + 8 + np.where(rows_k <= lds_rows, 0, 16)
+    per_row = 5 * 6 + 4 + 24 + 8 + np.where(rows_k <= lds_rows, 0, 16)
     tot_hbm = float(np.sum(it * rows_k * per_row))
     gbs_hbm = tot_hbm / (ms32 * 1e-3) / 1e9
     res["fp32_inner_graph"] = {
@@ -923,7 +925,8 @@ def batched_section(spk, count=4096, seed=42, smin=32, smax=256,
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "hbm_model_GBs": round(gbs_hbm, 1),
         "hbm_model_frac_peak": round(gbs_hbm / HBM_PEAK_GBS, 4),
-        "hbm_model": "per inner iteration and row: 40 B sliced-ELL matrix + "
+        "hbm_model": "per inner iteration and row: 30 B sliced-ELL matrix "
+                     "(FP32 values, 16-bit column offsets) + "
                      "4 B q + 24 B d/r pass + 8 B p pass, +16 B of p for "
                      f"systems above {lds_rows} rows (p in LDS below)"}
     c64 = spk.BatchConfig(1e-8, 1e-8, 3, 20000, spk.F64)

@@ -132,8 +132,11 @@ struct BatchArgs {
   // sliced-ELL copy of the inner-precision matrix (ell_w slots per row,
   // slot-major within each system: slot k of local row i of the system
   // starting at global row r0 (n rows) at r0 * ell_w + k * n + i; short
-  // rows padded with (own row, 0)), used when ell_w > 0
+  // rows padded with (own row, 0)), used when ell_w > 0.  Its columns are
+  // either global (ell_ci) or, when every one lies within 32767 of its row,
+  // 16-bit offsets from the row (ell_off; then ell_ci is null)
   const int32_t* __restrict__ ell_ci;
+  const int16_t* __restrict__ ell_off;
   const void* __restrict__ ell_v;
   int64_t ell_n;
   int ell_w;
@@ -180,7 +183,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[W], double* lds, int& phase) 
   phase ^= 1;
 }
 
-template <int BLOCK, typename T, int EW>
+template <int BLOCK, typename T, int EW, bool OFF16>
 __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
   constexpr int NW = BLOCK / 64;
   __shared__ double red[2 * NW * 2];
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
           if constexpr (EW > 0) {
             const int rb = (int)r0;
             const int32_t* __restrict__ cs = a.ell_ci + r0 * EW;
+            const int16_t* __restrict__ os = a.ell_off + r0 * EW;
             const T* __restrict__ vs = (const T*)a.ell_v + r0 * EW;
             for (int i = tid; i < n; i += U * BLOCK) {
               int32_t c[U][EW];
@@ -287,7 +291,8 @@ __global__ __launch_bounds__(BLOCK) void batch_pcg_kernel(BatchArgs a) {
                 if (o < n) {
   #pragma unroll
                   for (int k = 0; k < EW; ++k) {
-                    c[u][k] = ld32(cs, o) - rb;
+                    if constexpr (OFF16) c[u][k] = i + u * BLOCK + (int)ld32(os, o);
+                    else c[u][k] = ld32(cs, o) - rb;
                     v[u][k] = ld32(vs, o);
                     o += n;
                   }
@@ -474,7 +479,8 @@ __global__ void batch_ell_kernel(int64_t n, int w, const int64_t* __restrict__ s
                                  int K, const int32_t* __restrict__ rp,
                                  const int32_t* __restrict__ ci,
                                  const double* __restrict__ v,
-                                 int32_t* __restrict__ eci, T* __restrict__ ev) {
+                                 int32_t* __restrict__ eci, int16_t* __restrict__ eoff,
+                                 int* off_bad, T* __restrict__ ev) {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
        g += (int64_t)gridDim.x * blockDim.x) {
     int lo = 0, hi = K;  // largest k with sys_row[k] <= g
@@ -488,9 +494,15 @@ __global__ void batch_ell_kernel(int64_t n, int w, const int64_t* __restrict__ s
       const int64_t o = r0 * w + (int64_t)k * len_sys + (g - r0);
       if (k < len) {
         if (eci) eci[o] = ci[b + k];
+        if (eoff) {
+          const int64_t d = (int64_t)ci[b + k] - g;
+          if (d < -32768 || d > 32767) atomicOr(off_bad, 1);
+          else eoff[o] = (int16_t)d;
+        }
         ev[o] = (T)v[b + k];
       } else {
         if (eci) eci[o] = (int32_t)g;
+        if (eoff) eoff[o] = 0;
         ev[o] = (T)0;
       }
     }
@@ -517,6 +529,7 @@ struct spk_batch_s {
   // kEllMax entries); the FP64 values are made on first use
   int ell_w = 0;
   DevBuf<int32_t> ell_ci;
+  DevBuf<int16_t> ell_off;  // when every column is within 32767 of its row
   DevBuf<float> ell_v32;
   DevBuf<double> ell_v64;
   DevBuf<int32_t> iters, outer, conv;
@@ -603,13 +616,22 @@ struct spk_batch_s {
       w = ell_width_for(w);
       ell_w = w;
       ell_ci.alloc(n * (size_t)w);
+      ell_off.alloc(n * (size_t)w);
       ell_v32.alloc(n * (size_t)w);
+      DevBuf<int> obad(1);
+      SPK_HIP_TRY(hipMemsetAsync(obad.p, 0, 4, s));
       hipLaunchKernelGGL(batch_ell_kernel<float>, dim3(std::min<unsigned>(nblk(total), 16384)),
                          dim3(256), 0, s, total, w, (const int64_t*)sys_row.p, (int)K,
                          (const int32_t*)A->rp.p,
                          (const int32_t*)A->ci.p, (const double*)A->v64.p, ell_ci.p,
-                         ell_v32.p);
+                         ell_off.p, obad.p, ell_v32.p);
       SPK_HIP_TRY(hipGetLastError());
+      int ob = 0;
+      SPK_HIP_TRY(hipMemcpyAsync(&ob, obad.p, 4, hipMemcpyDeviceToHost, s));
+      SPK_HIP_TRY(hipStreamSynchronize(s));
+      // 16-bit offsets when they all fit (2 bytes per slot instead of 4)
+      if (ob) ell_off.release();
+      else ell_ci.release();
     }
   }
 
@@ -628,7 +650,8 @@ struct spk_batch_s {
                          stream(), total, ell_w, (const int64_t*)sys_row.p, (int)K,
                          (const int32_t*)A->rp.p,
                          (const int32_t*)A->ci.p, (const double*)A->v64.p,
-                         (int32_t*)nullptr, ell_v64.p);
+                         (int32_t*)nullptr, (int16_t*)nullptr, (int*)nullptr,
+                         ell_v64.p);
       SPK_HIP_TRY(hipGetLastError());
     }
   }
@@ -664,6 +687,7 @@ struct spk_batch_s {
     a.ell_w = ell_w;
     a.ell_n = total;
     a.ell_ci = ell_ci.p;
+    a.ell_off = ell_off.p;
     a.ell_v = prec == SPK_F32 ? (const void*)ell_v32.p : (const void*)ell_v64.p;
     a.counter = counter.p;
     a.iters = iters.p;
@@ -692,8 +716,20 @@ struct spk_batch_s {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(nblk(std::max<int64_t>(reset_rows, 1))),
                        dim3(256), 0, s, reset_rows, x, counter.p);
     const int grid = std::max(1, t.wg_per_cu * ctx().num_cus);
-#define SPK_BATCH_EW(BS, TT, EW) \
-  hipLaunchKernelGGL((batch_pcg_kernel<BS, TT, EW>), dim3(grid), dim3(BS), dyn, s, a)
+#define SPK_BATCH_EW(BS, TT, EW)                                                    \
+  do {                                                                              \
+    if constexpr (EW > 0) {                                                         \
+      if (a.ell_off)                                                                \
+        hipLaunchKernelGGL((batch_pcg_kernel<BS, TT, EW, true>), dim3(grid),        \
+                           dim3(BS), dyn, s, a);                                    \
+      else                                                                          \
+        hipLaunchKernelGGL((batch_pcg_kernel<BS, TT, EW, false>), dim3(grid),       \
+                           dim3(BS), dyn, s, a);                                    \
+    } else {                                                                        \
+      hipLaunchKernelGGL((batch_pcg_kernel<BS, TT, 0, false>), dim3(grid), dim3(BS), \
+                         dyn, s, a);                                                \
+    }                                                                               \
+  } while (0)
 #define SPK_BATCH_LAUNCH(BS, TT)                     \
   do {                                               \
     switch (a.ell_w) {                               \

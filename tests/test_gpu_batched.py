@@ -205,3 +205,44 @@ def test_every_row_width(O, spk, kind):
                 _, sr = O.cg(S, bk, tol=1e-9, restart_iterations=3000,
                              max_iterations=3000, left="jacobi")
                 assert abs(it[k] - sr.computed_iterations) <= 2, (kind, k)
+
+
+def test_wide_band_system_keeps_32_bit_columns(O, spk):
+    """The sliced-ELL copy stores each column as a 16-bit offset from its
+    row when every column lies within 32767 of its row (the diffusion
+    batches above); a system coupling rows 40,000 apart keeps the 32-bit
+    columns.  Both solve to the oracle's Jacobi-PCG: same counts within the
+    true-residual step, same x."""
+    n1 = 40001
+    rows, cols, vals = [], [], []
+    for i in range(n1):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n1:
+                rows.append(i); cols.append(j); vals.append(3.0 if i == j else -1.0)
+    rows += [0, n1 - 1]; cols += [n1 - 1, 0]; vals += [-0.5, -0.5]
+    W = O.make_matrix(n1, n1, rows, cols, vals)
+    D = O.batched_diffusion(11, 0, 20, 20)
+    for blocks in ((W, D), (D,)):
+        n = sum(B.nrows for B in blocks)
+        rp, ci, v, off = [np.zeros(1, np.int64)], [], [], [0]
+        base = 0
+        for B in blocks:
+            rp.append(B.row_ptr[1:] + rp[-1][-1])
+            ci.append(B.col_ind + off[-1])
+            v.append(B.vals)
+            off.append(off[-1] + B.nrows)
+        A = spk.CsrMatrix.from_host(n, n, np.concatenate(rp), np.concatenate(ci),
+                                    np.concatenate(v))
+        batch = spk.BatchedSystems.from_blocks(A, off)
+        b = spk.uniform_pm1(9, n)
+        x = spk.DeviceVector.zeros(n)
+        s = batch.solve(spk.BatchConfig(1e-8, 1e-8, 1, 20000, spk.F64), b, x)
+        assert s["converged"] == len(blocks)
+        it = batch.results()[0]
+        bh, xh = b.numpy(), x.numpy()
+        for k, B in enumerate(blocks):
+            bk, xk = bh[off[k]:off[k + 1]], xh[off[k]:off[k + 1]]
+            xr, sr = O.cg(B, bk, tol=1e-8, restart_iterations=20000,
+                          max_iterations=20000, left="jacobi")
+            assert abs(it[k] - sr.computed_iterations) <= 2, k
+            assert np.linalg.norm(xk - xr) <= 1e-6 * np.linalg.norm(xr)
