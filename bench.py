@@ -337,6 +337,9 @@ def mixed_precision_section(spk, grid=128, seed=42):
            "target_true_relres": 1e-10}
     runs = [(0, "cgs2", spk.ORTHO_CGS2), (0, "mgs", spk.ORTHO_MGS)]
     runs += [(k, "cgs2", spk.ORTHO_CGS2) for k in (2, 3, 4)]
+    # classical Gram-Schmidt once (half the passes over the basis): the
+    # FP64 true residual of the refinement decides convergence
+    runs += [(k, "cgs1", spk.ORTHO_CGS1) for k in (2, 3, 4)]
     timer = spk.Timer()
     for k, oname, ortho in runs:
         F.jacobi_sweeps = k

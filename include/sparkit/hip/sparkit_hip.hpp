@@ -408,7 +408,8 @@ struct Gmres_config {
   size_type restart_dimension{};
   size_type max_iterations{};
   bool collect_residuals{};
-  // not in the reference: SPK_ORTHO_MGS (its algorithm) or SPK_ORTHO_CGS2
+  // not in the reference: SPK_ORTHO_MGS (its algorithm), SPK_ORTHO_CGS2 or
+  // SPK_ORTHO_CGS1
   int orthogonalization{SPK_ORTHO_MGS};
 };
 template <typename T>

@@ -355,14 +355,20 @@ typedef struct {
   int32_t collect_residuals;
   int32_t precision;
   /* (F)GMRES only: SPK_ORTHO_MGS (0, the reference's modified Gram-Schmidt,
-   * gmres.hpp:258-267) or SPK_ORTHO_CGS2 (classical Gram-Schmidt twice:
+   * gmres.hpp:258-267), SPK_ORTHO_CGS2 (classical Gram-Schmidt twice:
    * the same Krylov basis up to rounding, nine launches per Arnoldi step
-   * instead of 2(j+2)+1) */
+   * instead of 2(j+2)+1) or SPK_ORTHO_CGS1 (no reference counterpart:
+   * classical Gram-Schmidt once, half CGS2's passes over the basis; the
+   * basis then keeps its orthogonality only as far as the problem's
+   * conditioning allows, and the residual estimate the convergence test
+   * reads can drift from the true residual -- for use where a true residual
+   * decides, as under spk_ir_solve's FP64 refinement) */
   int32_t orthogonalization;
   int32_t reserved;
 } spk_solver_config;
 #define SPK_ORTHO_MGS 0
 #define SPK_ORTHO_CGS2 1
+#define SPK_ORTHO_CGS1 2
 
 /* Mirrors CGSummary / Gmres_summary / ... (conjugate_gradient.hpp:26-40).
  * iteration_residuals go to the caller's `history` buffer (host, at least
@@ -475,7 +481,7 @@ typedef struct {
   int64_t restart;        /* inner restart dimension / restart_iterations */
   int64_t max_inner;      /* inner iteration budget per refinement step */
   int32_t inner_method;
-  int32_t orthogonalization; /* inner (F)GMRES: SPK_ORTHO_MGS / SPK_ORTHO_CGS2 */
+  int32_t orthogonalization; /* inner (F)GMRES: SPK_ORTHO_MGS / _CGS2 / _CGS1 */
 } spk_ir_config;
 
 typedef struct {
@@ -655,7 +661,7 @@ typedef struct {
   double stall_factor;             /* in (0, 1] */
   int32_t stall_cycles;            /* >= 1 */
   int32_t fallback;                /* SPK_FALLBACK_* */
-  int32_t orthogonalization;       /* FGMRES: SPK_ORTHO_MGS or SPK_ORTHO_CGS2 */
+  int32_t orthogonalization;       /* FGMRES: SPK_ORTHO_MGS, _CGS2 or _CGS1 */
   int32_t pad;
 } spk_learned_config;
 typedef struct {

@@ -28,7 +28,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS2, ORTHO_MGS, SPAI_LOSS_INVERSE,
+from ._lib import (F32, F64, BREAKDOWN, ORTHO_CGS1, ORTHO_CGS2, ORTHO_MGS,
+                   SPAI_LOSS_INVERSE,
                    SPAI_LOSS_SYMMETRIC, InvalidArgument,
                    LogicError, ReadError, SpkError, call, load)
 
@@ -42,7 +43,7 @@ __all__ = [
     "true_relres", "dot", "axpby", "inner_cg", "inner_ir", "InnerCg", "learned_solve", "LearnedConfig",
     "LearnedSummary", "FALLBACK_AUTO", "FALLBACK_JACOBI_PCG",
     "FALLBACK_ILU_GMRES", "FALLBACK_NONE",
-    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "SPAI_LOSS_INVERSE", "SPAI_LOSS_SYMMETRIC", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
+    "F32", "F64", "ORTHO_MGS", "ORTHO_CGS2", "ORTHO_CGS1", "SPAI_LOSS_INVERSE", "SPAI_LOSS_SYMMETRIC", "SpkError", "InvalidArgument", "LogicError", "ReadError", "Timer", "gather_rate",
     "IrConfig", "IrSummary", "ir_solve", "IR_GMRES", "IR_FGMRES", "IR_CG",
     "IR_BICGSTAB", "IR_CG_RELIABLE",
     "SpaiModel", "spai_features", "SPAI_NPARAM", "BatchConfig",
@@ -755,7 +756,7 @@ class CGConfig:
 class GmresConfig:
     """Gmres_config (gmres.hpp:24-47) + the arithmetic precision of the
     Krylov vectors and the orthogonalization (ORTHO_MGS, the reference's,
-    or ORTHO_CGS2)."""
+    ORTHO_CGS2, or ORTHO_CGS1 where a true residual decides)."""
     tolerance: float = 0.0
     restart_dimension: int = 0
     max_iterations: int = 0

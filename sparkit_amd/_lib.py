@@ -24,6 +24,7 @@ F32 = 1
 
 ORTHO_MGS = 0
 ORTHO_CGS2 = 1
+ORTHO_CGS1 = 2
 SPAI_LOSS_INVERSE = 0    # (1/n) ||I - M A||_F^2
 SPAI_LOSS_SYMMETRIC = 1  # (1/n) ||I - S A||_F^2, S = (M + M^T) / 2
 

@@ -189,9 +189,12 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
                                         (size_t)std::max<int64_t>(cfg->max_iterations, 1))
                       : nullptr;
   SPK_REQUIRE(cfg->orthogonalization == SPK_ORTHO_MGS ||
-                  cfg->orthogonalization == SPK_ORTHO_CGS2,
+                  cfg->orthogonalization == SPK_ORTHO_CGS2 ||
+                  cfg->orthogonalization == SPK_ORTHO_CGS1,
               SPK_INVALID_ARGUMENT, "unknown orthogonalization");
-  const bool cgs2 = cfg->orthogonalization == SPK_ORTHO_CGS2;
+  // classical Gram-Schmidt, twice (CGS2) or once (CGS1)
+  const bool cgs2 = cfg->orthogonalization != SPK_ORTHO_MGS;
+  const int npass = cfg->orthogonalization == SPK_ORTHO_CGS1 ? 1 : 2;
   SPK_REQUIRE(!cgs2 || m < 4096, SPK_INVALID_ARGUMENT,
               "classical Gram-Schmidt supports restart dimensions below 4096");
   double* ctmp = (double*)ws.get(kWsGmres + 6, sizeof(double) * (size_t)(m + 1));
@@ -291,13 +294,13 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
         const int jp1 = (int)(j + 1);
         const int nch = (int)((n + kDotChunk - 1) / kDotChunk);
         const size_t dyn = sizeof(double) * (size_t)jp1;
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < npass; ++pass) {
           hipLaunchKernelGGL((multi_dot_kernel<T>), dim3(nch, jp1), dim3(256), 0, s,
                              n, (const T*)V, nn, (const T*)vj1, nch, parts, stop);
           hipLaunchKernelGGL(multi_finish_kernel, dim3(jp1), dim3(256), 0, s, nch,
                              (const double*)parts, pass == 0 ? hcol : ctmp,
                              pass == 0 ? (double*)nullptr : hcol, stop);
-          if (pass == 0) {
+          if (pass + 1 < npass) {
             hipLaunchKernelGGL((multi_update_kernel<T, false>), dim3(vec_grid(n)),
                                dim3(256), dyn, s, n, (const T*)V, nn, jp1,
                                (const double*)hcol, vj1, parts, stop);
@@ -305,7 +308,8 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
             const int g = vec_grid(n);
             hipLaunchKernelGGL((multi_update_kernel<T, true>), dim3(g), dim3(256),
                                dyn, s, n, (const T*)V, nn, jp1,
-                               (const double*)ctmp, vj1, parts, stop);
+                               (const double*)(pass == 0 ? hcol : ctmp), vj1, parts,
+                               stop);
             const int jj = (int)j;
             launch_finish<1>(parts, g, [=] __device__(const double* t) {
               arnoldi_finish(S, jj, sqrt(t[0]));

@@ -120,7 +120,8 @@ extern "C" spk_status spk_learned_solve(spk_csr A, spk_precond M,
     SPK_REQUIRE(cfg->fallback >= SPK_FALLBACK_AUTO && cfg->fallback <= SPK_FALLBACK_NONE,
                 SPK_INVALID_ARGUMENT, "unknown fallback");
     SPK_REQUIRE(cfg->orthogonalization == SPK_ORTHO_MGS ||
-                    cfg->orthogonalization == SPK_ORTHO_CGS2,
+                    cfg->orthogonalization == SPK_ORTHO_CGS2 ||
+                    cfg->orthogonalization == SPK_ORTHO_CGS1,
                 SPK_INVALID_ARGUMENT, "unknown orthogonalization");
     *out = spk_learned_summary{};
     out->learned_relres = std::numeric_limits<double>::quiet_NaN();

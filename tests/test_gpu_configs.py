@@ -118,6 +118,29 @@ def test_config3_fp32_refinement_exact_ilu0(spk):
     assert np.linalg.norm(a - r) / np.linalg.norm(r) < 1e-6
 
 
+def test_config3_fp32_refinement_jacobi_ilu0_cgs1(spk):
+    """The fastest configuration-3 form: FP32-inner GMRES(50) with the
+    3-sweep Jacobi ILU(0) apply and single-pass classical Gram-Schmidt,
+    under FP64 refinement: the 1e-10 FP64 true residual in the inner-step
+    count of the CGS2 form (within a few steps), and the same solution."""
+    print("config 3: convdiff_3d(128) FP32-inner refinement, Jacobi ILU, CGS1",
+          flush=True)
+    A, _ = _convdiff_128(spk)
+    F = spk.ilu0(A)
+    F.jacobi_sweeps = 3
+    b = spk.uniform_pm1(42, A.nrows)
+    out = []
+    for ortho in (spk.ORTHO_CGS2, spk.ORTHO_CGS1):
+        x = spk.DeviceVector.zeros(A.nrows)
+        cfg = spk.IrConfig(1e-10, 20, 1e-4, 50, 2000, spk.IR_GMRES, ortho)
+        s = spk.ir_solve(b, x, cfg, A, F)
+        assert s.converged and spk.true_relres(A, b, x) < 1e-10
+        out.append((s.inner_iterations, x.numpy()))
+    assert abs(out[0][0] - out[1][0]) <= 5, (out[0][0], out[1][0])
+    a, r = out[1][1], out[0][1]
+    assert np.linalg.norm(a - r) / np.linalg.norm(r) < 1e-6
+
+
 @pytest.fixture(scope="module")
 def learned_1024(spk):
     """The network trained as the benchmark trains it: 512 matrices of
