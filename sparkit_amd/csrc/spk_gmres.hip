@@ -56,11 +56,29 @@ __device__ void arnoldi_finish(GmState* st, int j, double h) {
   double* g = st->g;
   col[j + 1] = h;
   st->hnorm = h;
-  for (int i = 0; i < j; ++i) {
-    double temp = cs[i] * col[i] + sn[i] * col[i + 1];
-    col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
-    col[i] = temp;
+  // the earlier rotations in the reference's order; only the rotated
+  // entry carries from one to the next, so each batch of eight loads its
+  // rotations and column entries together (one round trip, not eight)
+  double ci = col[0];
+  for (int i0 = 0; i0 < j; i0 += 8) {
+    const int nb = j - i0 < 8 ? j - i0 : 8;
+    double c8[8], s8[8], h8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < nb) {
+        c8[u] = cs[i0 + u];
+        s8[u] = sn[i0 + u];
+        h8[u] = col[i0 + u + 1];
+      }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < nb) {
+        const double temp = c8[u] * ci + s8[u] * h8[u];
+        ci = -s8[u] * ci + c8[u] * h8[u];
+        col[i0 + u] = temp;
+      }
   }
+  if (j > 0) col[j] = ci;
   double a = col[j], b = col[j + 1];
   if (b == 0.0) {
     cs[j] = 1.0;
@@ -152,19 +170,46 @@ __global__ __launch_bounds__(256) void multi_update_kernel(
   }
 }
 
-// form_solution's back-substitution R y = g (gmres.hpp:333-342)
-__global__ void back_substitute_kernel(GmState* st) {
-  if (st->done || threadIdx.x != 0) return;
+// form_solution's back-substitution R y = g (gmres.hpp:333-342), one
+// thread in the reference's order.  Up to kBackLds columns the block first
+// stages the triangle and g in LDS, so the serial solve reads no global
+// memory (a 50-column solve from global memory costs ~125 us of dependent
+// load latency).
+constexpr int kBackLds = 64;
+__global__ __launch_bounds__(256) void back_substitute_kernel(GmState* st) {
+  __shared__ double R[kBackLds * kBackLds];
+  __shared__ double gs[kBackLds], ys[kBackLds];
+  if (st->done) return;
   const int64_t ld = st->m + 1;
   const int nj = st->j_final + 1;
   const double* H = st->H;
   double* y = st->y;
-  for (int ii = nj; ii > 0; --ii) {
-    int i = ii - 1;
-    double yi = st->g[i];
-    for (int k = i + 1; k < nj; ++k) yi -= H[i + (int64_t)k * ld] * y[k];
-    y[i] = yi / H[i + (int64_t)i * ld];
+  if (nj > kBackLds) {
+    if (threadIdx.x != 0) return;
+    for (int ii = nj; ii > 0; --ii) {
+      int i = ii - 1;
+      double yi = st->g[i];
+      for (int k = i + 1; k < nj; ++k) yi -= H[i + (int64_t)k * ld] * y[k];
+      y[i] = yi / H[i + (int64_t)i * ld];
+    }
+    return;
   }
+  for (int t = threadIdx.x; t < nj * nj; t += blockDim.x) {
+    const int i = t % nj, k = t / nj;  // column k, row i: consecutive in H
+    R[i * kBackLds + k] = H[i + (int64_t)k * ld];
+  }
+  for (int t = threadIdx.x; t < nj; t += blockDim.x) gs[t] = st->g[t];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ii = nj; ii > 0; --ii) {
+      int i = ii - 1;
+      double yi = gs[i];
+      for (int k = i + 1; k < nj; ++k) yi -= R[i * kBackLds + k] * ys[k];
+      ys[i] = yi / R[i * kBackLds + i];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nj; t += blockDim.x) y[t] = ys[t];
 }
 
 template <typename T>
@@ -360,7 +405,7 @@ void gmres_solve(bool flexible, spk_csr_s* A, spk_precond_s* ML,
       }
     }
     // form_solution (gmres.hpp:328-354)
-    hipLaunchKernelGGL(back_substitute_kernel, dim3(1), dim3(64), 0, s, S);
+    hipLaunchKernelGGL(back_substitute_kernel, dim3(1), dim3(256), 0, s, S);
     const T* Zc = Z;
     launch_map(n, [=] __device__(int64_t e) {
       const int nj = S->j_final + 1;
