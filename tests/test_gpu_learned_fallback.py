@@ -45,6 +45,25 @@ def test_learned_path_converges_without_fallback(spk):
     assert np.array_equal(x.numpy(), x2.numpy())
 
 
+def test_learned_path_with_single_pass_gram_schmidt(spk):
+    """The guarded solve with classical Gram-Schmidt once (SPK_ORTHO_CGS1):
+    its FGMRES is the same as outside the guard, and the FP64 true residual
+    still decides (the estimate may drift with a less orthogonal basis)."""
+    A, b = _aniso(spk)
+    M = _trained(spk).preconditioner(A)
+    x = spk.DeviceVector.zeros(A.nrows)
+    cfg = spk.LearnedConfig(tolerance=1e-8, orthogonalization=spk.ORTHO_CGS1)
+    s = spk.learned_solve(b, x, cfg, A, M)
+    assert s.converged and spk.true_relres(A, b, x) < 1e-8
+    assert abs(spk.true_relres(A, b, x) - s.relres) <= 1e-3 * s.relres
+    if not s.fell_back:
+        x2 = spk.DeviceVector.zeros(A.nrows)
+        s2 = spk.fgmres(b, x2, spk.FgmresConfig(1e-8, 30, 50000, False, spk.F64,
+                                                spk.ORTHO_CGS1), A, M)
+        assert s2.computed_iterations == s.learned_iterations
+        assert np.array_equal(x.numpy(), x2.numpy())
+
+
 def test_nan_output_layer_falls_back_to_jacobi_pcg(spk):
     A, b = _aniso(spk)
     model = _trained(spk)
